@@ -1,0 +1,113 @@
+/*
+ * C ABI of the MI355X BASS/BIST superpixel library (libstspix.so).
+ *
+ * All array arguments are device pointers (HBM), row-major, channels-last:
+ *   image  float32 [H, W, 3]
+ *   flow   float32 [H, W, 2]   (u = dx, v = dy; flow of frame t-1 -> t)
+ *   labels int32   [H, W]
+ * Every call is asynchronous on the given hipStream_t except where a host
+ * value is returned (nspix_out, niters_state), in which case the call waits
+ * for the stream.  Calls on one context are not thread-safe; distinct
+ * contexts may be used from distinct threads.  No C++ exception crosses the
+ * boundary: every entry point returns an stsp_status code.
+ */
+#ifndef STSPIX_H
+#define STSPIX_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  STSP_OK = 0,
+  STSP_ERR_ARG = -1,      /* bad argument (null pointer, bad size) */
+  STSP_ERR_CAPACITY = -2, /* output capacity smaller than the result */
+  STSP_ERR_FILL = -3,     /* hole filling stopped shrinking (reference exits) */
+  STSP_ERR_HIP = -4,      /* a HIP runtime call failed */
+  STSP_ERR_STATE = -5     /* stream step without a previous frame */
+} stsp_status;
+
+/* Algorithm settings; defaults are the reference's (pyapi.cu:63-73,
+ * demo.cu:47-71).  Mirrors bist_forward's arguments plus the constants the
+ * reference hard-codes. */
+typedef struct {
+  int niters;         /* outer iterations of frame 0 (bist_forward niters) */
+  int niters_seg;     /* sweeps per iteration (pyapi.cu:63, 4) */
+  int sm_start;       /* split/merge only when idx > sm_start (0) */
+  int sp_size;        /* superpixel size (bist_forward sp_size) */
+  int sm_period;      /* split at idx%p==0, merge at idx%p==p/2 (4) */
+  int prop_niters;    /* >0: iterations of propagated frames; 0: adaptive */
+  float sigma2_app;   /* bist_forward sigma2_app */
+  float potts;        /* bist_forward potts */
+  float alpha_hastings; /* bist_forward alpha (log alpha) */
+  float split_alpha;  /* bist_forward split_alpha */
+  float merge_alpha;  /* pyapi.cu:65, 0 */
+  float thresh_relabel; /* pyapi.cu:68, 1e-5 */
+  float thresh_new;   /* pyapi.cu:69, 1e-2 */
+} stsp_config;
+
+/* Parameter arrays of one frame: the field set of the reference
+ * SuperpixelParams (structs.h:13-48), device pointers, cap entries each
+ * (x3 / x2 where noted). */
+typedef struct {
+  float* mu_app;              /* 3*cap */
+  float* prior_mu_app;        /* 3*cap */
+  double* mu_shape;           /* 2*cap */
+  double* sigma_shape;        /* 3*cap, inverse covariance (xx, xy, yy) */
+  float* logdet_sigma_shape;  /* cap */
+  double* prior_mu_shape;     /* 2*cap */
+  double* prior_sigma_shape;  /* 3*cap */
+  double* sample_sigma_shape; /* 3*cap */
+  int* counts;                /* cap */
+  int* sm_counts;             /* cap */
+  float* prior_counts;        /* cap */
+  int* ids;                   /* cap */
+  float* invalid_counts;      /* cap */
+  int cap;
+} stsp_params;
+
+typedef struct stsp_ctx stsp_ctx;
+
+/* Default configuration (reference values). */
+void stsp_default_config(stsp_config* cfg);
+
+/* A context owns the device workspace of one frame size; it replaces the
+ * per-call cudaMalloc/cudaFree of run_bass / run_prop (bass.cu:233-264,
+ * prop.cu:694-716). */
+int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** out);
+void stsp_ctx_destroy(stsp_ctx* ctx);
+int stsp_ctx_set_config(stsp_ctx* ctx, const stsp_config* cfg);
+
+/* Capacity an output parameter set needs: BASS 30*K0 (bass.cu:245),
+ * propagated frame 10*nspix_prev (prop.cu:702). */
+int stsp_bass_capacity(int sp_size, int height, int width);
+int stsp_prop_capacity(int nspix_prev);
+
+/* Single-image BASS: replaces run_bass (bass.cu:221-346).  Writes labels and
+ * compacted parameters; *nspix_out = number of superpixels. */
+int stsp_bass(stsp_ctx* ctx, const float* img, int* spix_out, stsp_params* out,
+              int* nspix_out, hipStream_t stream);
+
+/* One propagated frame: replaces shift_and_fill + run_prop
+ * (shift_and_fill.cu:108-339, prop.cu:677-865) as called from main_loop
+ * (pyapi.cu:128-170).  prev->prior_counts and prev->invalid_counts are
+ * updated in place, as the reference does.  *niters_state receives the
+ * adaptive iteration count (pyapi.cu:141-149). */
+int stsp_prop(stsp_ctx* ctx, const float* img, const float* flow, const int* spix_prev,
+              stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
+              int* nspix_out, int* niters_state, hipStream_t stream);
+
+/* Border overlay of get_marked_video (pyapi.cu:221-303,
+ * find_border_pixels_end seg_utils.cu:218-257): marked = clamp(vid,0,1)
+ * except label-border pixels, which take color[3]. nframes images. */
+int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,
+                      int height, int width, float* marked, hipStream_t stream);
+
+const char* stsp_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,98 @@
+/*
+ * CPU oracle for the streaming BASS/BIST superpixel path -- TEST INFRASTRUCTURE.
+ *
+ * This is a sequential C restatement of the reference algorithm
+ * (st_spix/csrc/bist/ *.cu of gauenk/st_spix), used only by tests/, by the
+ * smoke check and by the CPU-baseline leg of bench.py.  It is never linked
+ * into, loaded by or called from the product library.  Where the reference
+ * is order-nondeterministic (float atomics, first-arrival atomics, atomicAdd
+ * ID allocation) the oracle implements the documented deterministic rule
+ * that the GPU implementation also follows (rules R1..R15 in the source).
+ */
+#ifndef BIST_ORACLE_H
+#define BIST_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int niters;          /* outer iterations (frame 0 / BASS) */
+  int niters_seg;      /* boundary sweeps per iteration (reference: 4) */
+  int sm_start;        /* split/merge only for idx > sm_start (reference: 0) */
+  int sp_size;
+  int sm_period;       /* split at idx%p==0, merge at idx%p==p/2 (reference: 4) */
+  int prop_niters;     /* >0: fixed iterations for propagated frames; 0: adaptive 12/8/4 */
+  float sigma2_app;
+  float potts;
+  float alpha_hastings; /* log alpha */
+  float split_alpha;
+  float merge_alpha;
+  float thresh_relabel;
+  float thresh_new;
+} orc_cfg;
+
+/* SoA parameter set, field-for-field the reference SuperpixelParams
+ * (structs.h:13-48).  Arrays have nspix entries (x3 / x2 where noted). */
+typedef struct {
+  float* mu_app;             /* 3n */
+  float* prior_mu_app;       /* 3n */
+  double* mu_shape;          /* 2n */
+  double* sigma_shape;       /* 3n, inverse covariance */
+  float* logdet_sigma_shape; /* n */
+  double* prior_mu_shape;    /* 2n */
+  double* prior_sigma_shape; /* 3n */
+  double* sample_sigma_shape;/* 3n */
+  int* counts;               /* n */
+  int* sm_counts;            /* n */
+  float* prior_counts;       /* n */
+  int* ids;                  /* n */
+  float* invalid_counts;     /* n */
+} orc_params;
+
+int orc_nspix_from_spsize(int sp_size, int width, int height);
+void orc_init_hex_bruteforce(int* seg, int sp_size, int width, int height);
+void orc_init_hex(int* seg, int sp_size, int width, int height);
+void orc_simple_lut(unsigned char* out256);
+double orc_det_log(double x);
+
+/* per-label statistics: out[k*11 + j] = count, S_r, S_g, S_b, X, Y, XX, XY,
+ * YY, Q_r, Q_g (Q_b in out2[k]) -- see orc_label_stats for the layout. */
+void orc_label_stats(const float* img, const int* seg, int npix, int width,
+                     int nlabels, int64_t* out /* nlabels x 12 */);
+
+/* One update_seg call (niters_seg sweeps) with parameters given per label. */
+void orc_update_seg(const float* img, int* seg, int height, int width,
+                    const float* mu_app, const double* mu_shape,
+                    const double* sigma_shape, const double* logdet,
+                    const int* counts, int niters_seg, float sigma2_app,
+                    float potts);
+
+/* Single-image BASS (run_bass, bass.cu:221-346).  Writes labels and the
+ * compacted parameters (capacity cap entries).  Returns nspix or <0. */
+int orc_bass(const float* img, int height, int width, const orc_cfg* cfg,
+             int* seg_out, orc_params* out, int cap);
+
+/* One propagated frame (shift_and_fill + run_prop, pyapi.cu:128-170).
+ * prev: parameters of frame t-1 (nspix_prev entries; prior_counts and
+ * invalid_counts are updated in place as the reference does).
+ * niters_state: in/out adaptive iteration count (pyapi.cu:141-149).
+ * shifted_out / filled_out may be NULL.  Returns nspix or <0. */
+int orc_prop(const float* img, const float* flow, int height, int width,
+             const int* spix_prev, orc_params* prev, int nspix_prev,
+             const orc_cfg* cfg, int* niters_state, int* seg_out,
+             orc_params* out, int cap, int* shifted_out, int* filled_out);
+
+/* Pieces of shift_and_fill exposed for unit checks. */
+void orc_shift_labels(const int* spix, const float* flow_ds, const int* sizes,
+                      int height, int width, int* shifted);
+int orc_fill_missing(int* seg, const double* centers, int height, int width,
+                     int break_iter);
+void orc_invalidate_disconnected(int* seg, int height, int width, int nspix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

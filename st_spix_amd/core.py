@@ -1,0 +1,177 @@
+"""Torch-facing host code over the C ABI: contexts, parameter sets and the
+per-frame calls.  PyTorch supplies device memory and the current stream; all
+compute runs in libstspix.so.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, Optional, Tuple
+
+import torch
+
+from . import _lib
+
+Params = Dict[str, torch.Tensor]
+
+_DTYPES = {"float32": torch.float32, "float64": torch.float64, "int32": torch.int32}
+
+
+def _check_input(t: torch.Tensor, name: str) -> None:
+    # same contract as the reference's CHECK_INPUT (pyapi.cu:5-7)
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def make_config(niters: int = 500, sp_size: int = 25, potts: float = 0.5,
+                sigma2_app: float = 8.1e-5, alpha: float = math.log(0.5),
+                split_alpha: float = 0.0, merge_alpha: float = 0.0, niters_seg: int = 4,
+                sm_start: int = 0, sm_period: int = 4, prop_niters: int = 0,
+                thresh_relabel: float = 1e-5, thresh_new: float = 1e-2) -> _lib.Config:
+    return _lib.Config(niters=niters, niters_seg=niters_seg, sm_start=sm_start,
+                       sp_size=sp_size, sm_period=sm_period, prop_niters=prop_niters,
+                       sigma2_app=sigma2_app, potts=potts, alpha_hastings=alpha,
+                       split_alpha=split_alpha, merge_alpha=merge_alpha,
+                       thresh_relabel=thresh_relabel, thresh_new=thresh_new)
+
+
+class Context:
+    """Owns the device workspace for one frame size (stsp_ctx)."""
+
+    def __init__(self, height: int, width: int, cfg: _lib.Config, device=None):
+        self.H, self.W = int(height), int(width)
+        self.device = torch.device(device if device is not None else "cuda")
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().stsp_ctx_create(ctypes.byref(cfg), self.H, self.W,
+                                                  ctypes.byref(h)), "stsp_ctx_create")
+        self.handle = h
+
+    def set_config(self, cfg: _lib.Config) -> None:
+        self.cfg = cfg
+        _lib.check(_lib.lib().stsp_ctx_set_config(self.handle, ctypes.byref(cfg)),
+                   "stsp_ctx_set_config")
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().stsp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX: Dict[tuple, Context] = {}
+
+
+def get_context(height: int, width: int, cfg: _lib.Config, device) -> Context:
+    dev = torch.device(device)
+    key = (int(height), int(width), dev.index)
+    ctx = _CTX.get(key)
+    if ctx is None:
+        ctx = Context(height, width, cfg, dev)
+        _CTX[key] = ctx
+    else:
+        ctx.set_config(cfg)
+    return ctx
+
+
+def alloc_params(cap: int, device) -> Params:
+    out = {}
+    for name, dt, w in _lib.PARAM_FIELDS:
+        shape = (cap, w) if w > 1 else (cap,)
+        out[name] = torch.zeros(shape, dtype=_DTYPES[dt], device=device)
+    return out
+
+
+def params_struct(p: Params, cap: Optional[int] = None) -> _lib.ParamsStruct:
+    kw = {}
+    n = None
+    for name, dt, w in _lib.PARAM_FIELDS:
+        t = p[name]
+        if t.dtype != _DTYPES[dt]:
+            raise RuntimeError(f"params['{name}'] must be {dt}")
+        _check_input(t, f"params['{name}']")
+        n = t.shape[0] if n is None else min(n, t.shape[0])
+        kw[name] = t.data_ptr()
+    return _lib.ParamsStruct(cap=int(cap if cap is not None else n), **kw)
+
+
+def trim(p: Params, n: int) -> Params:
+    return {k: v[:n] for k, v in p.items()}
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def bass_frame(img: torch.Tensor, cfg: _lib.Config) -> Tuple[torch.Tensor, Params]:
+    """Single-image BASS on one (H, W, 3) float32 CUDA image."""
+    _check_input(img, "img")
+    if img.dtype != torch.float32 or img.dim() != 3 or img.shape[2] != 3:
+        raise RuntimeError("img must be float32 (H, W, 3)")
+    H, W = img.shape[:2]
+    ctx = get_context(H, W, cfg, img.device)
+    cap = _lib.lib().stsp_bass_capacity(cfg.sp_size, H, W)
+    out = alloc_params(cap, img.device)
+    spix = torch.empty((H, W), dtype=torch.int32, device=img.device)
+    n = ctypes.c_int(0)
+    with torch.cuda.device(img.device):
+        rc = _lib.lib().stsp_bass(ctx.handle, img.data_ptr(), spix.data_ptr(),
+                                  ctypes.byref(params_struct(out, cap)), ctypes.byref(n),
+                                  _stream())
+    _lib.check(rc, "stsp_bass")
+    return spix, trim(out, n.value)
+
+
+def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
+               params_prev: Params, cfg: _lib.Config) -> Tuple[torch.Tensor, Params, int]:
+    """One propagated frame.  ``params_prev['prior_counts']`` and
+    ``params_prev['invalid_counts']`` are updated in place, as the reference's
+    shift_and_fill does.  Returns (spix, params, adaptive_niters)."""
+    for t, name in ((img, "img"), (flow, "flow"), (spix_prev, "spix_prev")):
+        _check_input(t, name)
+    H, W = img.shape[:2]
+    if flow.shape != (H, W, 2) or flow.dtype != torch.float32:
+        raise RuntimeError("flow must be float32 (H, W, 2)")
+    if spix_prev.shape != (H, W) or spix_prev.dtype != torch.int32:
+        raise RuntimeError("spix_prev must be int32 (H, W)")
+    nprev = int(params_prev["ids"].shape[0])
+    ctx = get_context(H, W, cfg, img.device)
+    cap = _lib.lib().stsp_prop_capacity(nprev)
+    out = alloc_params(cap, img.device)
+    spix = torch.empty((H, W), dtype=torch.int32, device=img.device)
+    n = ctypes.c_int(0)
+    ns = ctypes.c_int(0)
+    with torch.cuda.device(img.device):
+        rc = _lib.lib().stsp_prop(ctx.handle, img.data_ptr(), flow.data_ptr(),
+                                  spix_prev.data_ptr(),
+                                  ctypes.byref(params_struct(params_prev, nprev)), nprev,
+                                  spix.data_ptr(), ctypes.byref(params_struct(out, cap)),
+                                  ctypes.byref(n), ctypes.byref(ns), _stream())
+    _lib.check(rc, "stsp_prop")
+    return spix, trim(out, n.value), ns.value
+
+
+def marked_video(vid: torch.Tensor, spix: torch.Tensor, color) -> torch.Tensor:
+    _check_input(vid, "vid")
+    _check_input(spix, "spix")
+    T, H, W = spix.shape
+    if vid.shape[:3] != spix.shape or vid.shape[3] != 3:
+        raise RuntimeError("vid must be (T, H, W, 3) matching spix (T, H, W)")
+    c = torch.as_tensor(color, dtype=torch.float32).flatten().cpu()
+    if c.numel() != 3:
+        raise RuntimeError("color must have 3 entries")
+    cbuf = (ctypes.c_float * 3)(*[float(v) for v in c])
+    marked = torch.empty_like(vid)
+    with torch.cuda.device(vid.device):
+        rc = _lib.lib().stsp_marked_video(vid.data_ptr(), spix.data_ptr(), cbuf, T, H, W,
+                                          marked.data_ptr(), _stream())
+    _lib.check(rc, "stsp_marked_video")
+    return marked

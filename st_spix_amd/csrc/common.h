@@ -1,0 +1,105 @@
+// Device-side definitions shared by the BASS/BIST kernels (gfx950).
+//
+// Layout in HBM (one frame workspace per stream):
+//   per pixel   : labels i32, border u8, shifted/halves/d1/d2/list/parent i32
+//   per label   : structure-of-arrays parameters (cap slots), int64 fixed-point
+//                 sufficient statistics, and a 48-byte "sweep record" that packs
+//                 exactly what the boundary update reads for one candidate label.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "detmath.h"
+
+namespace stsp {
+
+// Fixed-point scales of the deterministic reductions: colour and colour
+// squares use 2^-32, flow uses 2^-24.  Integer addition is associative, so
+// every accumulation order (tiles, atomics, batching) gives the same bits.
+constexpr double kFxColor = 4294967296.0;
+constexpr double kFxFlow = 16777216.0;
+
+constexpr int kBlock = 256;
+
+struct alignas(16) SweepRec {
+  float mu[3];
+  float logdet;  // logdet_sigma_shape narrowed to float
+  float isig[3]; // inverse shape covariance narrowed to float (xx, xy, yy)
+  int count;
+  double mx, my; // mean position (double: the update truncates x - mu to int)
+};
+
+// Per-label sufficient statistics.  S = colour sums, Q = colour square sums
+// (2^-32 fixed point), positions exact.
+struct LabelStats {
+  int* count;    // [cap]
+  int64_t* S;    // [cap*3]
+  int64_t* Q;    // [cap*3]
+  int64_t* X;    // [cap*5]  X, Y, XX, XY, YY
+  int* ninvalid; // [cap]
+};
+
+// Working parameters (the reference's spix_params, structure of arrays).
+struct Params {
+  float* mu_app;           // 3
+  float* prior_mu_app;     // 3
+  double* mu_shape;        // 2
+  double* sigma_shape;     // 3 (inverse covariance)
+  double* prior_mu_shape;  // 2
+  double* prior_sigma_shape;   // 3
+  double* sample_sigma_shape;  // 3
+  double* prior_icov;      // 3
+  double* logdet;          // 1
+  int* count;
+  float* icount;
+  int* sm_count;
+  float* prior_count;
+  int* valid;
+  int* prop;
+};
+
+// Split/merge helper state, 2*cap+2 slots.
+struct SmState {
+  int* count;
+  int64_t* S;      // 3 per slot
+  int64_t* Q;      // 3 per slot
+  int* ninvalid;
+  int* count_f;
+  double* b_n;     // 3
+  double* b_n_f;   // 3
+  double* marg;    // own marginal
+  double* hasting;
+  int* merge;
+  int* remove;
+  int* select;
+  int* has_bn;
+  int* pairs;      // 2 per slot
+  int* winner;     // per target: max accepting k
+  int* accept;
+  int* rank;
+  int* valid0;
+  float* pc0;
+};
+
+struct NigConsts {
+  double a0, b0, logb0, logpi, log2, log_alpha;
+};
+
+__device__ __forceinline__ int64_t fx_color(float v) {
+  return __double2ll_rn((double)v * kFxColor);
+}
+__device__ __forceinline__ int64_t fx_square(float v) {
+  double d = (double)v;
+  return __double2ll_rn(d * d * kFxColor);
+}
+__device__ __forceinline__ double fx_to_double(int64_t s) {
+  return (double)s * (1.0 / kFxColor);
+}
+__device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
+  atomicAdd((unsigned long long*)p, (unsigned long long)v);
+}
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace stsp

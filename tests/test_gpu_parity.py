@@ -1,0 +1,154 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Labels and integer fields are compared bit-exactly; the
+floating-point parameters are also required to be bit-identical, because
+both sides evaluate the same IEEE operation sequence (no FMA contraction,
+deterministic log, exact integer statistics)."""
+import numpy as np
+import pytest
+import torch
+
+from st_spix_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["mu_app", "prior_mu_app", "mu_shape", "sigma_shape", "logdet_sigma_shape",
+          "prior_mu_shape", "prior_sigma_shape", "sample_sigma_shape", "counts",
+          "sm_counts", "prior_counts", "ids", "invalid_counts"]
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def assert_labels(gpu, ref, what):
+    g = gpu.cpu().numpy() if torch.is_tensor(gpu) else gpu
+    bad = np.argwhere(g != ref)
+    assert bad.size == 0, f"{what}: {len(bad)} label mismatches, first at {bad[0]} " \
+                          f"gpu={g[tuple(bad[0])]} ref={ref[tuple(bad[0])]}"
+
+
+def assert_params(gp, rp, what):
+    for f in FIELDS:
+        g = gp[f].cpu().numpy()
+        r = rp[f]
+        assert g.shape == r.shape, f"{what}.{f}: shape {g.shape} vs {r.shape}"
+        same = (g == r) | (np.isnan(g) & np.isnan(r)) if g.dtype.kind == "f" else (g == r)
+        if not same.all():
+            i = np.argwhere(~same)[0]
+            pytest.fail(f"{what}.{f}: {int((~same).sum())} mismatches, first {i}: "
+                        f"gpu={g[tuple(i)]!r} ref={r[tuple(i)]!r}")
+
+
+def gpu_bass(img, **kw):
+    from st_spix_amd import core
+    cfg = core.make_config(**kw)
+    return core.bass_frame(cuda(img), cfg)
+
+
+def orc_cfg(orc, **kw):
+    m = dict(kw)
+    if "alpha" in m:
+        m["alpha_hastings"] = m.pop("alpha")
+    return orc.make_cfg(**m)
+
+
+@pytest.mark.parametrize("H,W,sp,niters,nreg,seed", [
+    (48, 64, 8, 24, 8, 2), (33, 47, 6, 20, 5, 3), (60, 90, 12, 40, 10, 4),
+    (2, 2, 1, 3, 1, 0), (17, 5, 3, 9, 2, 1)])
+def test_bass_parity_small(orc, H, W, sp, niters, nreg, seed):
+    img = synth.voronoi_image(H, W, nreg, seed=seed)
+    kw = dict(niters=niters, sp_size=sp, potts=10.0)
+    spix, params = gpu_bass(img, **kw)
+    torch.cuda.synchronize()
+    seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
+    assert_labels(spix, seg_r, "labels")
+    assert_params(params, p_r, "params")
+
+
+def test_bass_parity_config0(orc):
+    """The reference's single-image case: 321x481, 40 Voronoi regions, sp 25,
+    potts 10, 500 iterations with split/merge."""
+    img = synth.voronoi_image(321, 481, 40, noise=0.03, seed=0)
+    kw = dict(niters=500, sp_size=25, potts=10.0)
+    spix, params = gpu_bass(img, **kw)
+    torch.cuda.synchronize()
+    seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
+    assert_labels(spix, seg_r, "config0 labels")
+    assert_params(params, p_r, "config0 params")
+
+
+def test_bass_deterministic():
+    img = synth.voronoi_image(96, 128, 12, seed=7)
+    a, pa = gpu_bass(img, niters=40, sp_size=10, potts=10.0)
+    b, pb = gpu_bass(img, niters=40, sp_size=10, potts=10.0)
+    assert torch.equal(a, b)
+    for f in FIELDS:
+        assert torch.equal(pa[f], pb[f]) or torch.allclose(pa[f], pb[f], equal_nan=True)
+
+
+@pytest.mark.parametrize("prop_niters", [0, 12])
+def test_prop_parity_video(orc, prop_niters):
+    from st_spix_amd import core
+    T, H, W = 5, 64, 80
+    vid, flows = synth.voronoi_video(T, H, W, nregions=10, vmax=3.0, seed=11)
+    kw = dict(niters=30, sp_size=8, potts=10.0, prop_niters=prop_niters)
+    cfg = core.make_config(**kw)
+    ocfg = orc_cfg(orc, **kw)
+    spix, params = core.bass_frame(cuda(vid[0]), cfg)
+    seg_r, p_r = orc.bass(vid[0], ocfg)
+    assert_labels(spix, seg_r, "frame0")
+    assert_params(params, p_r, "frame0")
+    for t in range(1, T):
+        spix, params, ns = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg)
+        seg_r, p_r, ns_r = orc.prop(vid[t], flows[t - 1], seg_r, p_r, ocfg)
+        torch.cuda.synchronize()
+        assert ns == ns_r
+        assert_labels(spix, seg_r, f"frame{t}")
+        assert_params(params, p_r, f"frame{t}")
+
+
+def test_bist_forward_matches_oracle(orc):
+    import st_spix_amd
+    T, H, W = 3, 48, 64
+    vid, flows = synth.voronoi_video(T, H, W, nregions=8, vmax=2.5, seed=5)
+    out = st_spix_amd.bist_forward(cuda(vid), cuda(flows), 20, 8, 10.0, 8.1e-5,
+                                   float(np.log(0.5)), 0.0, True, False)
+    ref = orc.bist(vid, flows, orc.make_cfg(niters=20, sp_size=8, potts=10.0))
+    assert_labels(out, ref, "bist_forward")
+    # image mode: every frame is independent BASS
+    out2 = st_spix_amd.bist_forward(cuda(vid), cuda(flows), 20, 8, 10.0, 8.1e-5,
+                                    float(np.log(0.5)), 0.0, False, False)
+    ref2 = orc.bist(vid, flows, orc.make_cfg(niters=20, sp_size=8, potts=10.0), video_mode=False)
+    assert_labels(out2, ref2, "bist_forward image mode")
+
+
+def test_marked_video():
+    import st_spix_amd
+    rng = np.random.default_rng(0)
+    vid = rng.uniform(-0.2, 1.2, (2, 9, 11, 3)).astype(np.float32)
+    spix = rng.integers(0, 3, (2, 9, 11)).astype(np.int32)
+    color = [0.0, 0.0, 1.0]
+    got = st_spix_amd.get_marked_video(cuda(vid), cuda(spix), color).cpu().numpy()
+    exp = np.clip(vid, 0, 1)
+    for t in range(2):
+        s = spix[t]
+        b = np.zeros_like(s, bool)
+        b[1:, :] |= s[1:, :] > s[:-1, :]
+        b[:-1, :] |= s[:-1, :] > s[1:, :]
+        b[:, 1:] |= s[:, 1:] > s[:, :-1]
+        b[:, :-1] |= s[:, :-1] > s[:, 1:]
+        exp[t][b] = color
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_prop_all_holes_raises():
+    """A flow that moves every label out of the frame leaves nothing to fill
+    from: the reference exits (fill_missing.cu:97-111); here the call raises."""
+    from st_spix_amd import _lib, core
+    img = synth.voronoi_image(32, 40, 4, seed=1)
+    cfg = core.make_config(niters=8, sp_size=8)
+    spix, params = core.bass_frame(cuda(img), cfg)
+    flow = np.full((32, 40, 2), 500.0, np.float32)
+    with pytest.raises(_lib.StspError) as e:
+        core.prop_frame(cuda(img), cuda(flow), spix, params, cfg)
+    assert e.value.code == -3
