@@ -80,6 +80,11 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
 void stsp_ctx_destroy(stsp_ctx* ctx);
 int stsp_ctx_set_config(stsp_ctx* ctx, const stsp_config* cfg);
 
+/* Launch shape of the fused boundary sweep: 0 = one sweep per launch
+ * (32x16 tiles), 1 = two (32x32), 2 = four (64x32).  Results are identical
+ * for every shape; only speed differs (chosen by the autotuner). */
+int stsp_ctx_set_tuning(stsp_ctx* ctx, int sweep_shape);
+
 /* Capacity an output parameter set needs: BASS 30*K0 (bass.cu:245),
  * propagated frame 10*nspix_prev (prop.cu:702). */
 int stsp_bass_capacity(int sp_size, int height, int width);
@@ -104,6 +109,13 @@ int stsp_prop(stsp_ctx* ctx, const float* img, const float* flow, const int* spi
  * except label-border pixels, which take color[3]. nframes images. */
 int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,
                       int height, int width, float* marked, hipStream_t stream);
+
+/* Device-time accounting of the hot kernels, measured with hipEvent pairs
+ * on the launch stream around each launch of the class (no counterpart in
+ * the reference, which prints host clock() times, demo.cu:243-340). */
+enum { STSP_KCLASS_SWEEP = 0, STSP_KCLASS_STATS = 1, STSP_KCLASS_COUNT = 2 };
+int stsp_ctx_enable_timing(stsp_ctx* ctx, int on);
+int stsp_ctx_kernel_time(stsp_ctx* ctx, int kclass, double* total_ms, long long* launches);
 
 const char* stsp_status_string(int status);
 

@@ -59,7 +59,8 @@ _lib = None
 # every symbol include/stspix.h declares
 EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_set_config", "stsp_bass_capacity", "stsp_prop_capacity", "stsp_bass",
-           "stsp_prop", "stsp_marked_video", "stsp_status_string"]
+           "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
+           "stsp_ctx_kernel_time", "stsp_ctx_set_tuning"]
 
 
 def lib() -> ctypes.CDLL:
@@ -83,6 +84,10 @@ def lib() -> ctypes.CDLL:
         L.stsp_prop.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ParamsStruct), i, vp,
                                 ctypes.POINTER(ParamsStruct), pi, pi, vp]
         L.stsp_marked_video.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float), i, i, i, vp, vp]
+        L.stsp_ctx_enable_timing.argtypes = [vp, i]
+        L.stsp_ctx_kernel_time.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_longlong)]
+        L.stsp_ctx_set_tuning.argtypes = [vp, i]
         L.stsp_status_string.argtypes = [i]
         L.stsp_status_string.restype = ctypes.c_char_p
         _lib = L

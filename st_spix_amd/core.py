@@ -50,6 +50,21 @@ class Context:
                                                   ctypes.byref(h)), "stsp_ctx_create")
         self.handle = h
 
+    def set_tuning(self, sweep_shape: int) -> None:
+        _lib.check(_lib.lib().stsp_ctx_set_tuning(self.handle, int(sweep_shape)),
+                   "stsp_ctx_set_tuning")
+
+    def enable_timing(self, on: bool = True) -> None:
+        _lib.check(_lib.lib().stsp_ctx_enable_timing(self.handle, int(on)),
+                   "stsp_ctx_enable_timing")
+
+    def kernel_time(self, kclass: int):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_longlong(0)
+        _lib.check(_lib.lib().stsp_ctx_kernel_time(self.handle, kclass, ctypes.byref(ms),
+                                                   ctypes.byref(n)), "stsp_ctx_kernel_time")
+        return ms.value, n.value
+
     def set_config(self, cfg: _lib.Config) -> None:
         self.cfg = cfg
         _lib.check(_lib.lib().stsp_ctx_set_config(self.handle, ctypes.byref(cfg)),
@@ -68,6 +83,14 @@ class Context:
 
 
 _CTX: Dict[tuple, Context] = {}
+_SWEEP_SHAPE = {"value": 2}
+
+
+def set_sweep_shape(shape: int) -> None:
+    """Launch shape of the fused boundary sweep for all contexts (0, 1, 2)."""
+    _SWEEP_SHAPE["value"] = int(shape)
+    for ctx in _CTX.values():
+        ctx.set_tuning(int(shape))
 
 
 def get_context(height: int, width: int, cfg: _lib.Config, device) -> Context:
@@ -76,6 +99,7 @@ def get_context(height: int, width: int, cfg: _lib.Config, device) -> Context:
     ctx = _CTX.get(key)
     if ctx is None:
         ctx = Context(height, width, cfg, dev)
+        ctx.set_tuning(_SWEEP_SHAPE["value"])
         _CTX[key] = ctx
     else:
         ctx.set_config(cfg)

@@ -38,6 +38,7 @@ struct LabelStats {
   int64_t* Q;    // [cap*3]
   int64_t* X;    // [cap*5]  X, Y, XX, XY, YY
   int* ninvalid; // [cap]
+  int* bbox;     // [cap*4]  min x, max x, min y, max y (gathered on request)
 };
 
 // Working parameters (the reference's spix_params, structure of arrays).
@@ -98,6 +99,29 @@ __device__ __forceinline__ double fx_to_double(int64_t s) {
 }
 __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
   atomicAdd((unsigned long long*)p, (unsigned long long)v);
+}
+
+// Runs of equal keys inside one wave: returns true on the run head (lowest
+// lane of the run) and sets run_end to the first lane after the run.
+__device__ __forceinline__ bool wave_run(int key, int& run_end) {
+  const int lane = threadIdx.x & 63;
+  int kprev = __shfl_up(key, 1);
+  bool head = (lane == 0) || (kprev != key);
+  unsigned long long heads = __ballot(head);
+  unsigned long long above = heads & ~((2ull << lane) - 1ull);
+  run_end = above ? (__ffsll((long long)above) - 1) : 64;
+  return head;
+}
+
+// Sum of v over the lanes of this lane's run, valid on the run head.
+__device__ __forceinline__ int64_t run_sum64(int64_t v, int run_end) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int64_t w = __shfl_down(v, d);
+    if (lane + d < run_end) v += w;
+  }
+  return v;
 }
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }

@@ -77,14 +77,27 @@ struct Arena {
 
 }  // namespace
 
+// Event-pair timer for one kernel class (measures device time of the
+// launches it brackets on the launch stream).
+struct KTimer {
+  static const int kMax = 1024;
+  hipEvent_t ev[2 * kMax];
+  int created = 0, n = 0;
+  double ms = 0.0;
+  long long count = 0;
+};
+
 struct stsp_ctx {
   stsp_config cfg;
+  int timing = 0;
+  KTimer timers[STSP_KCLASS_COUNT];
   int H, W, npix;
   int cap = 0;    // label slots of the working parameter set
   int nprev = 0;  // label slots of the previous-frame arrays
   SimpleLut lut;
   // pixel buffers
-  int *seg, *halves, *d1, *d2, *list, *shifted, *parent, *rsize;
+  int *seg, *seg2, *halves, *d1, *d2, *list, *shifted, *parent, *rsize;
+  int sweep_shape = 2;  // 0: 1 sweep/launch, 1: 2, 2: 4 (fused)
   uint8_t* border;
   unsigned long long* keys;
   // label buffers
@@ -111,10 +124,11 @@ struct stsp_ctx {
 static int alloc_pixels(stsp_ctx* c) {
   size_t n = c->npix;
   Arena a;
-  a.size = 12 * 256 + n * (4 * 8 + 1 + 8) + 4096;
+  a.size = 13 * 256 + n * (4 * 9 + 1 + 8) + 4096;
   HIP_TRY(hipMalloc(&c->pix_mem, a.size));
   a.base = (char*)c->pix_mem;
   c->seg = a.take<int>(n);
+  c->seg2 = a.take<int>(n);
   c->halves = a.take<int>(n);
   c->d1 = a.take<int>(n);
   c->d2 = a.take<int>(n);
@@ -144,7 +158,7 @@ static int ensure_labels(stsp_ctx* c, int cap) {
   size_t n = cap, nsm = 2 * n + 2;
   // byte budget, generously rounded (each take pads to 256 B)
   size_t bytes = n * (3 * 4 * 2 + 8 * (2 + 3 + 2 + 3 + 3 + 3 + 1) + 4 * 6) +  // params
-                 n * (4 + 8 * 11 + 4) +                                        // st
+                 n * (4 + 8 * 11 + 4 + 16) +                                   // st
                  nsm * (4 + 8 * 11 + 4) +                                      // sst
                  nsm * (4 * 4 + 8 * 3 * 2 + 8 * 2 + 4 * 8) + nsm * 8 +          // sm
                  n * sizeof(SweepRec) + n * 4 * 8 + compact_tmp_bytes(cap) + 96 * 256;
@@ -173,11 +187,13 @@ static int ensure_labels(stsp_ctx* c, int cap) {
   c->st.Q = a.take<int64_t>(3 * n);
   c->st.X = a.take<int64_t>(5 * n);
   c->st.ninvalid = a.take<int>(n);
+  c->st.bbox = a.take<int>(4 * n);
   c->sst.count = a.take<int>(nsm);
   c->sst.S = a.take<int64_t>(3 * nsm);
   c->sst.Q = a.take<int64_t>(3 * nsm);
   c->sst.X = a.take<int64_t>(5 * nsm);
   c->sst.ninvalid = a.take<int>(nsm);
+  c->sst.bbox = nullptr;
   SmState& sm = c->sm;
   sm.count = c->sst.count;
   sm.S = c->sst.S;
@@ -244,6 +260,40 @@ __global__ void k_set_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+static void tstart(stsp_ctx* c, int cls, hipStream_t s) {
+  if (!c->timing) return;
+  KTimer& t = c->timers[cls];
+  if (t.n >= KTimer::kMax) return;
+  if (t.created <= t.n) {
+    (void)hipEventCreate(&t.ev[2 * t.n]);
+    (void)hipEventCreate(&t.ev[2 * t.n + 1]);
+    t.created = t.n + 1;
+  }
+  (void)hipEventRecord(t.ev[2 * t.n], s);
+}
+static void tstop(stsp_ctx* c, int cls, hipStream_t s) {
+  if (!c->timing) return;
+  KTimer& t = c->timers[cls];
+  if (t.n >= KTimer::kMax) return;
+  (void)hipEventRecord(t.ev[2 * t.n + 1], s);
+  t.n++;
+}
+// called after the stream has been synchronised
+static void tflush(stsp_ctx* c) {
+  if (!c->timing) return;
+  for (int k = 0; k < STSP_KCLASS_COUNT; k++) {
+    KTimer& t = c->timers[k];
+    for (int i = 0; i < t.n; i++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]) == hipSuccess) {
+        t.ms += ms;
+        t.count++;
+      }
+    }
+    t.n = 0;
+  }
+}
+
 static int read_int(stsp_ctx* c, const int* dev, hipStream_t s, int* out) {
   HIP_TRY(hipMemcpyAsync(c->host, dev, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -251,8 +301,11 @@ static int read_int(stsp_ctx* c, const int* dev, hipStream_t s, int* out) {
   return STSP_OK;
 }
 
-static void update_params(stsp_ctx* c, const float* img, int nbuf, hipStream_t s) {
-  launch_stats(img, c->seg, nullptr, c->npix, c->W, c->st, nbuf, s);
+static void update_params(stsp_ctx* c, const float* img, int nbuf, hipStream_t s,
+                          bool bbox = false) {
+  tstart(c, STSP_KCLASS_STATS, s);
+  launch_stats(img, c->seg, nullptr, c->npix, c->W, c->st, nbuf, s, bbox);
+  tstop(c, STSP_KCLASS_STATS, s);
   launch_param_update(c->P, c->st, c->rec, nbuf, c->cfg.sp_size, 0, s);
 }
 
@@ -260,11 +313,21 @@ static void split_step(stsp_ctx* c, const float* img, const int* shifted, int nb
                        int direction, const NigConsts& nc, int prop_mode, hipStream_t s) {
   int nsm = 2 * nbuf + 2;
   launch_sm_clear(c->sm, c->sst, nsm, nbuf, s);
-  launch_split_proposal(c->seg, c->P, nbuf, c->max_spix, direction, c->W, c->H, c->d1, c->d2,
-                        c->halves, c->list, c->off, c->cursor, s);
+  launch_split_proposal(c->seg, c->P, c->st.bbox, nbuf, c->max_spix, direction, c->W, c->H,
+                        c->d1, c->d2, c->halves, s);
   launch_stats(img, c->halves, shifted, c->npix, c->W, c->sst, nsm, s);
   launch_split_eval_apply(c->seg, c->halves, c->P, c->st, c->sst, c->sm, nbuf, c->max_spix, nc,
                           c->cfg.split_alpha, prop_mode, c->npix, c->total, s);
+}
+
+// niters_seg boundary sweeps; the result lands in c->seg
+static void sweeps(stsp_ctx* c, const float* img, hipStream_t s) {
+  int* res = launch_sweeps(img, c->seg, c->seg2, c->rec, c->cfg.niters_seg, c->cfg.sigma2_app,
+                           c->cfg.potts, c->H, c->W, c->lut, c->sweep_shape, s);
+  if (res != c->seg) {
+    c->seg2 = c->seg;
+    c->seg = res;
+  }
 }
 
 static inline bool is_split_iter(const stsp_config& cfg, int idx) {
@@ -297,6 +360,10 @@ static int finish_frame(stsp_ctx* c, int nbuf, int prev_nspix, int* spix_out, st
   launch_export(c->P, o, c->total, c->alive, nspix, s);
   HIP_TRY(hipMemcpyAsync(spix_out, c->seg, sizeof(int) * c->npix, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
+  if (c->timing) {
+    HIP_TRY(hipStreamSynchronize(s));
+    tflush(c);
+  }
   return STSP_OK;
 }
 
@@ -375,11 +442,38 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
 void stsp_ctx_destroy(stsp_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
+  for (int k = 0; k < STSP_KCLASS_COUNT; k++)
+    for (int i = 0; i < 2 * c->timers[k].created; i++) (void)hipEventDestroy(c->timers[k].ev[i]);
   if (c->pix_mem) (void)hipFree(c->pix_mem);
   if (c->lab_mem) (void)hipFree(c->lab_mem);
   if (c->prev_mem) (void)hipFree(c->prev_mem);
   if (c->host) (void)hipHostFree(c->host);
   delete c;
+}
+
+int stsp_ctx_enable_timing(stsp_ctx* c, int on) {
+  if (!c) return STSP_ERR_ARG;
+  c->timing = on ? 1 : 0;
+  for (int k = 0; k < STSP_KCLASS_COUNT; k++) {
+    c->timers[k].ms = 0.0;
+    c->timers[k].count = 0;
+    c->timers[k].n = 0;
+  }
+  return STSP_OK;
+}
+
+int stsp_ctx_kernel_time(stsp_ctx* c, int kclass, double* total_ms, long long* launches) {
+  if (!c || kclass < 0 || kclass >= STSP_KCLASS_COUNT || !total_ms || !launches)
+    return STSP_ERR_ARG;
+  *total_ms = c->timers[kclass].ms;
+  *launches = c->timers[kclass].count;
+  return STSP_OK;
+}
+
+int stsp_ctx_set_tuning(stsp_ctx* c, int sweep_shape) {
+  if (!c || sweep_shape < 0 || sweep_shape > 2) return STSP_ERR_ARG;
+  c->sweep_shape = sweep_shape;
+  return STSP_OK;
 }
 
 int stsp_ctx_set_config(stsp_ctx* c, const stsp_config* cfg) {
@@ -410,19 +504,21 @@ int stsp_bass(stsp_ctx* c, const float* img, int* spix_out, stsp_params* out, in
   NigConsts nc = make_nig(cfg);
   int smc = 1;
   for (int idx = 0; idx < cfg.niters; idx++) {
-    update_params(c, img, nbuf, s);
+    update_params(c, img, nbuf, s, is_split_iter(cfg, idx));
     if (is_split_iter(cfg, idx)) {
       smc += 1;
       split_step(c, img, nullptr, nbuf, smc % 2 + 1, nc, 0, s);
       update_params(c, img, nbuf, s);
     }
     if (is_merge_iter(cfg, idx)) {
+      launch_border(c->seg, c->border, H, W, s);
       launch_merge(c->seg, c->border, c->P, c->st, c->sst, c->sm, nbuf, 2 * nbuf + 2, nc, 0.0f,
                    smc % 2, 0, img, W, H, s);
       update_params(c, img, nbuf, s);
     }
-    launch_update_seg(img, c->seg, c->border, c->rec, cfg.niters_seg, cfg.sigma2_app, cfg.potts,
-                      H, W, c->lut, s);
+    tstart(c, STSP_KCLASS_SWEEP, s);
+    sweeps(c, img, s);
+    tstop(c, STSP_KCLASS_SWEEP, s);
   }
   update_params(c, img, nbuf, s);
   return finish_frame(c, nbuf, 0, spix_out, out, nspix_out, s);
@@ -473,7 +569,7 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
   NigConsts nc = make_nig(cfg);
   int smc = 1;
   for (int idx = 0; idx < niters; idx++) {
-    update_params(c, img, nbuf, s);
+    update_params(c, img, nbuf, s, is_split_iter(cfg, idx));
     if (is_split_iter(cfg, idx)) {
       smc += 1;
       split_step(c, img, c->shifted, nbuf, smc % 2 + 1, nc, 1, s);
@@ -484,12 +580,14 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
                      cfg.thresh_relabel, cfg.thresh_new, c->alive, c->target, c->want_new,
                      c->rank, c->src_of, c->total, s);
       smc += 1;
+      launch_border(c->seg, c->border, H, W, s);
       launch_merge(c->seg, c->border, c->P, c->st, c->sst, c->sm, nbuf, 2 * nbuf + 2, nc,
                    cfg.merge_alpha, smc % 2, 1, img, W, H, s);
       update_params(c, img, nbuf, s);
     }
-    launch_update_seg(img, c->seg, c->border, c->rec, cfg.niters_seg, cfg.sigma2_app, cfg.potts,
-                      H, W, c->lut, s);
+    tstart(c, STSP_KCLASS_SWEEP, s);
+    sweeps(c, img, s);
+    tstop(c, STSP_KCLASS_SWEEP, s);
   }
   update_params(c, img, nbuf, s);
   return finish_frame(c, nbuf, n, spix_out, out, nspix_out, s);

@@ -196,11 +196,18 @@ __global__ void k_stats_clear(LabelStats st, int n) {
   }
 #pragma unroll
   for (int c = 0; c < 5; c++) st.X[5 * k + c] = 0;
+  if (st.bbox) {
+    st.bbox[4 * k] = 0x7fffffff;
+    st.bbox[4 * k + 1] = -1;
+    st.bbox[4 * k + 2] = 0x7fffffff;
+    st.bbox[4 * k + 3] = -1;
+  }
 }
 
 // One wave aggregates runs of equal labels along its 64 consecutive pixels
 // before touching global memory: each lane adds its pixel into the run head
 // through a segmented shuffle reduction, and only run heads issue atomics.
+template <bool BBOX>
 __global__ void k_stats_accum(const float* __restrict__ img, const int* __restrict__ seg,
                               const int* __restrict__ shifted, int npix, int W, LabelStats st,
                               int nlab) {
@@ -211,6 +218,7 @@ __global__ void k_stats_accum(const float* __restrict__ img, const int* __restri
 #pragma unroll
   for (int j = 0; j < 11; j++) v[j] = 0;
   int cnt = 0, ninv = 0;
+  int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
   if (t < npix) {
     k = seg[t];
     if (k >= nlab) k = -1;
@@ -230,6 +238,10 @@ __global__ void k_stats_accum(const float* __restrict__ img, const int* __restri
       v[10] = y * y;
       cnt = 1;
       if (shifted) ninv = shifted[t] < 0;
+      if (BBOX) {
+        bx0 = bx1 = (int)x;
+        by0 = by1 = (int)y;
+      }
     }
   }
   // segmented inclusive suffix reduction towards the run head (lowest lane of
@@ -249,11 +261,24 @@ __global__ void k_stats_accum(const float* __restrict__ img, const int* __restri
     int64_t w2[11];
 #pragma unroll
     for (int j = 0; j < 11; j++) w2[j] = __shfl_down(v[j], d);
+    int ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0;
+    if (BBOX) {
+      ax0 = __shfl_down(bx0, d);
+      ax1 = __shfl_down(bx1, d);
+      ay0 = __shfl_down(by0, d);
+      ay1 = __shfl_down(by1, d);
+    }
     if (lane + d < run_end) {
       cnt += c2;
       ninv += n2;
 #pragma unroll
       for (int j = 0; j < 11; j++) v[j] += w2[j];
+      if (BBOX) {
+        bx0 = min(bx0, ax0);
+        bx1 = max(bx1, ax1);
+        by0 = min(by0, ay0);
+        by1 = max(by1, ay1);
+      }
     }
   }
   if (head && k >= 0) {
@@ -266,13 +291,24 @@ __global__ void k_stats_accum(const float* __restrict__ img, const int* __restri
     }
 #pragma unroll
     for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * k + c, v[6 + c]);
+    if (BBOX) {
+      atomicMin(st.bbox + 4 * k, bx0);
+      atomicMax(st.bbox + 4 * k + 1, bx1);
+      atomicMin(st.bbox + 4 * k + 2, by0);
+      atomicMax(st.bbox + 4 * k + 3, by1);
+    }
   }
 }
 
 void launch_stats(const float* img, const int* seg, const int* shifted, int npix, int W,
-                  LabelStats st, int nlab, hipStream_t s) {
+                  LabelStats st, int nlab, hipStream_t s, bool bbox) {
+  if (!bbox) st.bbox = nullptr;
   k_stats_clear<<<div_up(nlab, kBlock), kBlock, 0, s>>>(st, nlab);
-  k_stats_accum<<<div_up(npix, kBlock), kBlock, 0, s>>>(img, seg, shifted, npix, W, st, nlab);
+  if (bbox)
+    k_stats_accum<true><<<div_up(npix, kBlock), kBlock, 0, s>>>(img, seg, shifted, npix, W, st, nlab);
+  else
+    k_stats_accum<false><<<div_up(npix, kBlock), kBlock, 0, s>>>(img, seg, shifted, npix, W, st,
+                                                                 nlab);
 }
 
 // ---------------------------------------------------------------------------

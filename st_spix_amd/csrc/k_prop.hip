@@ -21,12 +21,25 @@ __global__ void k_flow_clear(int64_t* fsum, int* counts, int n) {
 __global__ void k_flow_accum(const float* __restrict__ flow, const int* __restrict__ spix,
                              int npix, int64_t* fsum, int* counts, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  int L = spix[t];
-  if (L < 0 || L >= n) return;
-  atomic_add_i64(fsum + 2 * L, __double2ll_rn((double)flow[2 * t] * kFxFlow));
-  atomic_add_i64(fsum + 2 * L + 1, __double2ll_rn((double)flow[2 * t + 1] * kFxFlow));
-  atomicAdd(counts + L, 1);
+  int L = -1;
+  int64_t u = 0, v = 0;
+  if (t < npix) {
+    L = spix[t];
+    if (L >= n) L = -1;
+    if (L >= 0) {
+      u = __double2ll_rn((double)flow[2 * t] * kFxFlow);
+      v = __double2ll_rn((double)flow[2 * t + 1] * kFxFlow);
+    }
+  }
+  int run_end;
+  bool head = wave_run(L, run_end);
+  u = run_sum64(u, run_end);
+  v = run_sum64(v, run_end);
+  if (head && L >= 0) {
+    atomic_add_i64(fsum + 2 * L, u);
+    atomic_add_i64(fsum + 2 * L + 1, v);
+    atomicAdd(counts + L, run_end - (int)(threadIdx.x & 63));
+  }
 }
 
 __global__ void k_flow_norm(const int64_t* fsum, const int* counts, const double* mu_shape,
@@ -251,9 +264,14 @@ void launch_invalidate_disconnected(int* seg, int* parent, int* rsize, unsigned 
 // ---------------------------------------------------------------------------
 __global__ void k_count_labels(const int* __restrict__ seg, int npix, int* counts, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  int k = seg[t];
-  if (k >= 0 && k < n) atomicAdd(counts + k, 1);
+  int k = -1;
+  if (t < npix) {
+    k = seg[t];
+    if (k >= n) k = -1;
+  }
+  int run_end;
+  bool head = wave_run(k, run_end);
+  if (head && k >= 0) atomicAdd(counts + k, run_end - (int)(threadIdx.x & 63));
 }
 
 void launch_count_labels(const int* seg, int npix, int* counts, int n, hipStream_t s) {

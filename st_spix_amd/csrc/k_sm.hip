@@ -56,109 +56,136 @@ void launch_sm_clear(SmState sm, LabelStats sst, int nsm, int nbuf, hipStream_t 
 // ---------------------------------------------------------------------------
 // split proposal: seeds, per-label BFS, halves
 // ---------------------------------------------------------------------------
-// init_split(_p), split_merge.cu:803-845
-__global__ void k_split_seeds(Params P, int nbuf, const int* __restrict__ seg, int* d1, int* d2,
-                              int W, int H, int direction, const int* max_spix) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nbuf || *max_spix > nbuf / 2 || P.valid[k] == 0) return;
-  for (int which = 0; which < 2; which++) {
-    int dir = which == 0 ? direction : -direction;
-    int x, y;
-    if (dir == 1 || dir == -1) {
-      x = (int)(P.mu_shape[2 * k]) + dir;
-      y = (int)(P.mu_shape[2 * k + 1]);
-    } else {
-      x = (int)(P.mu_shape[2 * k]);
-      y = (int)(P.mu_shape[2 * k + 1]) + dir;
-    }
-    int ind = y * W + x;
-    if (ind < 0 || ind > W * H - 1) continue;
-    if (seg[ind] != k) continue;
-    (which == 0 ? d1 : d2)[ind] = 1;
+// Seed pixel of label k for one direction (init_split(_p),
+// split_merge.cu:803-845): (int)mu +- direction, kept only if it carries k.
+__device__ __forceinline__ int split_seed(const Params& P, const int* __restrict__ seg, int k,
+                                          int dir, int W, int H) {
+  int x, y;
+  if (dir == 1 || dir == -1) {
+    x = (int)(P.mu_shape[2 * k]) + dir;
+    y = (int)(P.mu_shape[2 * k + 1]);
+  } else {
+    x = (int)(P.mu_shape[2 * k]);
+    y = (int)(P.mu_shape[2 * k + 1]) + dir;
   }
+  int ind = y * W + x;
+  if (ind < 0 || ind > W * H - 1) return -1;
+  return seg[ind] == k ? ind : -1;
 }
 
-// counting sort of pixels by label: list[off[k] .. off[k]+count[k])
-__global__ void k_list_scatter(const int* __restrict__ seg, int npix, const int* __restrict__ off,
-                               int* cursor, int* list, int nlab) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  int k = seg[t];
-  if (k < 0 || k >= nlab) return;
-  int pos = off[k] + atomicAdd(cursor + k, 1);
-  list[pos] = t;
+__global__ void k_copy_i32(const int* __restrict__ a, int* __restrict__ b, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
-// Level-synchronous 4-connected BFS restricted to one label, from both seeds
-// at once (calc_split_candidate, split_merge.cu:757-800).  One workgroup per
-// label walks the label's pixel list; distances live in the global d1/d2 maps
-// (only this workgroup touches pixels of its label).
-__global__ void __launch_bounds__(256) k_bfs(const int* __restrict__ list,
-                                             const int* __restrict__ off,
-                                             const int* __restrict__ cnt,
-                                             const int* __restrict__ seg, int* d1, int* d2,
-                                             int W, int H, int nbuf, const int* max_spix,
-                                             const int* valid) {
-  int k = blockIdx.x;
-  if (k >= nbuf || *max_spix > nbuf / 2 || valid[k] == 0) return;
-  int n = cnt[k];
-  if (n <= 0) return;
-  const int* L = list + off[k];
+// Two-seed split proposal of one label (calc_split_candidate, split_merge.cu:
+// 757-800 run to convergence from both seeds, then calc_seg_split 847-860).
+// One workgroup per label runs the level-synchronous 4-connected BFS inside
+// the label's bounding box with both distance maps in LDS; a label whose box
+// exceeds the LDS budget runs the same levels on the global d1/d2 maps.
+// Pixels with d1 > d2 (unreached = -1) go to half k + max_spix + 1.
+constexpr int kBfsCap = 8192;
+
+__global__ void __launch_bounds__(256)
+    k_bfs_box(const int* __restrict__ seg, Params P, const int* __restrict__ bbox, int nbuf,
+              const int* __restrict__ max_spix, int direction, int W, int H, int* d1g, int* d2g,
+              int* halves) {
+  const int k = blockIdx.x;
+  const int ms = *max_spix;
+  if (k >= nbuf || ms > nbuf / 2 || P.valid[k] == 0 || P.count[k] <= 0) return;
+  const int bx0 = bbox[4 * k], bx1 = bbox[4 * k + 1], by0 = bbox[4 * k + 2], by1 = bbox[4 * k + 3];
+  if (bx1 < bx0 || by1 < by0) return;
+  const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1, nc = bw * bh;
+  const int s1 = split_seed(P, seg, k, direction, W, H);
+  const int s2 = split_seed(P, seg, k, -direction, W, H);
+  const int k2 = k + ms + 1;
+  __shared__ short d1[kBfsCap], d2[kBfsCap];
+  __shared__ uint8_t m[kBfsCap];
   __shared__ int changed;
+  const int tid = threadIdx.x;
+  if (nc <= kBfsCap) {
+    for (int i = tid; i < nc; i += 256) {
+      int gx = bx0 + i % bw, gy = by0 + i / bw, g = gy * W + gx;
+      m[i] = seg[g] == k;
+      d1[i] = (g == s1) ? 1 : -1;
+      d2[i] = (g == s2) ? 1 : -1;
+    }
+    __syncthreads();
+    for (int level = 1;; level++) {
+      if (tid == 0) changed = 0;
+      __syncthreads();
+      int ch = 0;
+      for (int i = tid; i < nc; i += 256) {
+        if (!m[i]) continue;
+        bool u1 = d1[i] == -1, u2 = d2[i] == -1;
+        if (!u1 && !u2) continue;
+        int lx = i % bw, ly = i / bw;
+        bool h1 = false, h2 = false;
+        if (ly > 0 && m[i - bw]) { h1 |= d1[i - bw] == level; h2 |= d2[i - bw] == level; }
+        if (lx > 0 && m[i - 1]) { h1 |= d1[i - 1] == level; h2 |= d2[i - 1] == level; }
+        if (ly < bh - 1 && m[i + bw]) { h1 |= d1[i + bw] == level; h2 |= d2[i + bw] == level; }
+        if (lx < bw - 1 && m[i + 1]) { h1 |= d1[i + 1] == level; h2 |= d2[i + 1] == level; }
+        if (u1 && h1) { d1[i] = (short)(level + 1); ch = 1; }
+        if (u2 && h2) { d2[i] = (short)(level + 1); ch = 1; }
+      }
+      if (ch) changed = 1;
+      __syncthreads();
+      const int go = changed;
+      __syncthreads();  // every thread has read `changed` before it is reset
+      if (!go) break;
+    }
+    for (int i = tid; i < nc; i += 256) {
+      if (m[i] && d1[i] > d2[i]) halves[(by0 + i / bw) * W + bx0 + i % bw] = k2;
+    }
+    return;
+  }
+  // large label: same levels with the distance maps in global memory
+  for (int i = tid; i < nc; i += 256) {
+    int g = (by0 + i / bw) * W + bx0 + i % bw;
+    if (seg[g] != k) continue;
+    d1g[g] = (g == s1) ? 1 : -1;
+    d2g[g] = (g == s2) ? 1 : -1;
+  }
+  __syncthreads();
   for (int level = 1;; level++) {
-    if (threadIdx.x == 0) changed = 0;
+    if (tid == 0) changed = 0;
     __syncthreads();
     int ch = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      int p = L[i];
-      int x = p % W, y = p / W;
-      int a = d1[p], b = d2[p];
-      if (a != level && b != level) continue;
-      int nb[4] = {y > 0 ? p - W : -1, x > 0 ? p - 1 : -1, y < H - 1 ? p + W : -1,
-                   x < W - 1 ? p + 1 : -1};
+    for (int i = tid; i < nc; i += 256) {
+      int lx = i % bw, ly = i / bw, g = (by0 + ly) * W + bx0 + lx;
+      if (seg[g] != k) continue;
+      bool u1 = d1g[g] == -1, u2 = d2g[g] == -1;
+      if (!u1 && !u2) continue;
+      bool h1 = false, h2 = false;
+      int nb[4] = {ly > 0 ? g - W : -1, lx > 0 ? g - 1 : -1, ly < bh - 1 ? g + W : -1,
+                   lx < bw - 1 ? g + 1 : -1};
       for (int j = 0; j < 4; j++) {
         int q = nb[j];
         if (q < 0 || seg[q] != k) continue;
-        if (a == level && d1[q] == -1) {
-          d1[q] = level + 1;
-          ch = 1;
-        }
-        if (b == level && d2[q] == -1) {
-          d2[q] = level + 1;
-          ch = 1;
-        }
+        h1 |= d1g[q] == level;
+        h2 |= d2g[q] == level;
       }
+      if (u1 && h1) { d1g[g] = level + 1; ch = 1; }
+      if (u2 && h2) { d2g[g] = level + 1; ch = 1; }
     }
     if (ch) changed = 1;
     __syncthreads();
-    if (!changed) break;
+    const int go = changed;
     __syncthreads();
+    if (!go) break;
+  }
+  for (int i = tid; i < nc; i += 256) {
+    int g = (by0 + i / bw) * W + bx0 + i % bw;
+    if (seg[g] == k && d1g[g] > d2g[g]) halves[g] = k2;
   }
 }
 
-// calc_seg_split, split_merge.cu:847-860
-__global__ void k_halves(const int* __restrict__ seg, const int* __restrict__ d1,
-                         const int* __restrict__ d2, int* halves, int npix, const int* max_spix) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  int v = seg[t];
-  if (d1[t] > d2[t]) v += (*max_spix + 1);
-  halves[t] = v;
-}
-
-void launch_split_proposal(const int* seg, Params P, int nbuf, const int* max_spix, int direction,
-                           int W, int H, int* d1, int* d2, int* halves, int* list, int* off,
-                           int* cursor, hipStream_t s) {
+void launch_split_proposal(const int* seg, Params P, const int* bbox, int nbuf,
+                           const int* max_spix, int direction, int W, int H, int* d1, int* d2,
+                           int* halves, hipStream_t s) {
   int npix = W * H;
-  launch_fill(d1, -1, npix, s);
-  launch_fill(d2, -1, npix, s);
-  k_split_seeds<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, nbuf, seg, d1, d2, W, H, direction,
-                                                        max_spix);
-  launch_scan(P.count, off, nbuf, nullptr, s);
-  launch_fill(cursor, 0, nbuf, s);
-  k_list_scatter<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, npix, off, cursor, list, nbuf);
-  k_bfs<<<nbuf, 256, 0, s>>>(list, off, P.count, seg, d1, d2, W, H, nbuf, max_spix, P.valid);
-  k_halves<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, d1, d2, halves, npix, max_spix);
+  k_copy_i32<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, halves, npix);
+  k_bfs_box<<<nbuf, 256, 0, s>>>(seg, P, bbox, nbuf, max_spix, direction, W, H, d1, d2, halves);
 }
 
 // ---------------------------------------------------------------------------

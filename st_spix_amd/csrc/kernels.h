@@ -42,7 +42,7 @@ void launch_update_seg(const float* img, int* seg, uint8_t* border, const SweepR
                        int niters_seg, float sigma2_app, float potts, int H, int W,
                        const SimpleLut& lut, hipStream_t s);
 void launch_stats(const float* img, const int* seg, const int* shifted, int npix, int W,
-                  LabelStats st, int nlab, hipStream_t s);
+                  LabelStats st, int nlab, hipStream_t s, bool bbox = false);
 void launch_param_update(Params P, LabelStats st, SweepRec* rec, int nbuf, int sp_size,
                          int summary_mode, hipStream_t s);
 void launch_store_sample(Params P, LabelStats st, int nbuf, hipStream_t s);
@@ -60,9 +60,9 @@ void launch_export(Params P, OutParams o, const int* nspix, const int* alive, in
 
 // k_sm.hip
 void launch_sm_clear(SmState sm, LabelStats sst, int nsm, int nbuf, hipStream_t s);
-void launch_split_proposal(const int* seg, Params P, int nbuf, const int* max_spix, int direction,
-                           int W, int H, int* d1, int* d2, int* halves, int* list, int* off,
-                           int* cursor, hipStream_t s);
+void launch_split_proposal(const int* seg, Params P, const int* bbox, int nbuf,
+                           const int* max_spix, int direction, int W, int H, int* d1, int* d2,
+                           int* halves, hipStream_t s);
 void launch_split_eval_apply(int* seg, const int* halves, Params P, LabelStats st,
                              LabelStats sst, SmState sm, int nbuf, int* max_spix,
                              const NigConsts& nc, float split_alpha, int prop_mode, int npix,
@@ -98,4 +98,11 @@ void launch_relabel(int* seg, int npix, Params P, int nbuf, const float* prev_pr
 namespace stsp {
 void launch_marked(const float* vid, const int* spix, const float* color, int T, int H, int W,
                    float* marked, hipStream_t s);
+}
+
+namespace stsp {
+// k_sweep.hip: returns the buffer that holds the swept labels
+int* launch_sweeps(const float* img, int* seg_a, int* seg_b, const SweepRec* rec,
+                   int niters_seg, float sigma2_app, float potts, int H, int W,
+                   const SimpleLut& lut, int shape, hipStream_t s);
 }

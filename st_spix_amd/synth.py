@@ -14,6 +14,8 @@ with (u, v) = (dx, dy), as the reference's drop-in expects.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 from scipy.spatial import cKDTree
 
@@ -137,3 +139,53 @@ def batch_images(n: int = 32, lo=(240, 320), hi=(720, 1280), nregions: int = 40,
         W = int(rng.integers(lo[1], hi[1] + 1))
         out.append(voronoi_image(H, W, nregions, seed=seed * 1000 + i + 1))
     return out
+
+
+def voronoi_video_torch(T: int, H: int, W: int, nregions: int = 120, vmax: float = 4.0,
+                        noise: float = 0.03, flow_noise: float = 0.5, seed: int = 0,
+                        device="cuda"):
+    """Same motion model as voronoi_video, generated on the GPU with torch
+    (used by the benchmark for large frames).  Returns (vid, flows) on device."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    seeds = torch.stack([torch.rand(nregions, generator=g, dtype=torch.float64) * W,
+                         torch.rand(nregions, generator=g, dtype=torch.float64) * H], -1)
+    ang = torch.rand(nregions, generator=g, dtype=torch.float64) * 2 * math.pi
+    spd = torch.rand(nregions, generator=g, dtype=torch.float64) * vmax * 0.8
+    vel = torch.stack([spd * torch.cos(ang), spd * torch.sin(ang)], -1)
+    A = torch.randn(nregions, 2, 2, generator=g, dtype=torch.float64) * 0.002
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cols = torch.from_numpy(random_colors(rng, nregions)).to(device)
+    seeds, vel, A = seeds.to(device), vel.to(device), A.to(device)
+    yy, xx = torch.meshgrid(torch.arange(H, device=device, dtype=torch.float64),
+                            torch.arange(W, device=device, dtype=torch.float64), indexing="ij")
+    gd = torch.Generator(device=device).manual_seed(seed + 1)
+    vid = torch.empty((T, H, W, 3), dtype=torch.float32, device=device)
+    flows = torch.empty((max(T - 1, 0), H, W, 2), dtype=torch.float32, device=device)
+    for t in range(T):
+        lab = torch.empty((H, W), dtype=torch.long, device=device)
+        step = max(1, (1 << 24) // (W * nregions))
+        for r0 in range(0, H, step):
+            r1 = min(H, r0 + step)
+            dx = xx[r0:r1, :, None] - seeds[None, None, :, 0]
+            dy = yy[r0:r1, :, None] - seeds[None, None, :, 1]
+            lab[r0:r1] = torch.argmin(dx * dx + dy * dy, -1)
+        vid[t] = cols[lab] + noise * torch.randn((H, W, 3), generator=gd, device=device)
+        if t == T - 1:
+            break
+        dx = xx - seeds[lab, 0]
+        dy = yy - seeds[lab, 1]
+        u = vel[lab, 0] + A[lab, 0, 0] * dx + A[lab, 0, 1] * dy
+        v = vel[lab, 1] + A[lab, 1, 0] * dx + A[lab, 1, 1] * dy
+        mag = torch.sqrt(u * u + v * v)
+        scale = torch.clamp(vmax / torch.clamp(mag, min=1e-9), max=1.0)
+        flows[t, ..., 0] = (u * scale).float() + flow_noise * torch.randn(
+            (H, W), generator=gd, device=device)
+        flows[t, ..., 1] = (v * scale).float() + flow_noise * torch.randn(
+            (H, W), generator=gd, device=device)
+        seeds = seeds + vel
+        for c, lim in ((0, W), (1, H)):
+            out = (seeds[:, c] < 0) | (seeds[:, c] > lim - 1)
+            vel[out, c] *= -1
+            seeds[:, c] = seeds[:, c].clamp(0, lim - 1)
+    return vid, flows

@@ -77,6 +77,24 @@ def test_bass_parity_config0(orc):
     assert_params(params, p_r, "config0 params")
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_sweep_shapes_bit_identical(orc, shape):
+    """Every launch shape of the fused sweep (1, 2 or 4 sweeps per launch,
+    different tiles) gives the oracle's labels; odd sizes exercise ragged
+    tiles."""
+    from st_spix_amd import core
+    img = synth.voronoi_image(75, 101, 9, seed=6)
+    kw = dict(niters=16, sp_size=9, potts=10.0)
+    core.set_sweep_shape(shape)
+    try:
+        spix, params = gpu_bass(img, **kw)
+    finally:
+        core.set_sweep_shape(2)
+    seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
+    assert_labels(spix, seg_r, f"shape{shape}")
+    assert_params(params, p_r, f"shape{shape}")
+
+
 def test_bass_deterministic():
     img = synth.voronoi_image(96, 128, 12, seed=7)
     a, pa = gpu_bass(img, niters=40, sp_size=10, potts=10.0)
