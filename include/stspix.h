@@ -104,6 +104,21 @@ int stsp_prop(stsp_ctx* ctx, const float* img, const float* flow, const int* spi
               stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
               int* nspix_out, int* niters_state, hipStream_t stream);
 
+/* Batched (served) form: the refinement loops of several propagated frames,
+ * one per context, run as shared launches (one launch covers every frame;
+ * per-label work is pulled from work lists), optionally replayed from a
+ * captured hipGraph.  Each frame's result equals what stsp_prop returns for
+ * it alone.  No counterpart in the reference, whose frame loop is sequential
+ * (pyapi.cu:92-189).  Graph replay needs a non-default stream. */
+typedef struct stsp_batch stsp_batch;
+int stsp_batch_create(int max_frames, stsp_batch** out);
+void stsp_batch_destroy(stsp_batch* batch);
+int stsp_batch_use_graph(stsp_batch* batch, int on);
+int stsp_prop_batch(stsp_batch* batch, stsp_ctx** ctxs, int n, const float** imgs,
+                    const float** flows, const int** spix_prev, stsp_params* prevs,
+                    const int* nspix_prev, int** spix_out, stsp_params* outs, int* nspix_out,
+                    int* niters_state, hipStream_t stream);
+
 /* Border overlay of get_marked_video (pyapi.cu:221-303,
  * find_border_pixels_end seg_utils.cu:218-257): marked = clamp(vid,0,1)
  * except label-border pixels, which take color[3]. nframes images. */

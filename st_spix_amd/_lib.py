@@ -60,7 +60,8 @@ _lib = None
 EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_set_config", "stsp_bass_capacity", "stsp_prop_capacity", "stsp_bass",
            "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
-           "stsp_ctx_kernel_time", "stsp_ctx_set_tuning"]
+           "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
+           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_prop_batch"]
 
 
 def lib() -> ctypes.CDLL:
@@ -88,6 +89,14 @@ def lib() -> ctypes.CDLL:
         L.stsp_ctx_kernel_time.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_longlong)]
         L.stsp_ctx_set_tuning.argtypes = [vp, i]
+        L.stsp_batch_create.argtypes = [i, ctypes.POINTER(vp)]
+        L.stsp_batch_destroy.argtypes = [vp]
+        L.stsp_batch_destroy.restype = None
+        L.stsp_batch_use_graph.argtypes = [vp, i]
+        L.stsp_prop_batch.argtypes = [vp, ctypes.POINTER(vp), i, ctypes.POINTER(vp),
+                                      ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                      ctypes.POINTER(ParamsStruct), pi, ctypes.POINTER(vp),
+                                      ctypes.POINTER(ParamsStruct), pi, pi, vp]
         L.stsp_status_string.argtypes = [i]
         L.stsp_status_string.restype = ctypes.c_char_p
         _lib = L

@@ -135,13 +135,17 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def bass_frame(img: torch.Tensor, cfg: _lib.Config) -> Tuple[torch.Tensor, Params]:
+def bass_frame(img: torch.Tensor, cfg: _lib.Config,
+               ctx: Optional[Context] = None) -> Tuple[torch.Tensor, Params]:
     """Single-image BASS on one (H, W, 3) float32 CUDA image."""
     _check_input(img, "img")
     if img.dtype != torch.float32 or img.dim() != 3 or img.shape[2] != 3:
         raise RuntimeError("img must be float32 (H, W, 3)")
     H, W = img.shape[:2]
-    ctx = get_context(H, W, cfg, img.device)
+    if ctx is None:
+        ctx = get_context(H, W, cfg, img.device)
+    else:
+        ctx.set_config(cfg)
     cap = _lib.lib().stsp_bass_capacity(cfg.sp_size, H, W)
     out = alloc_params(cap, img.device)
     spix = torch.empty((H, W), dtype=torch.int32, device=img.device)
@@ -155,7 +159,8 @@ def bass_frame(img: torch.Tensor, cfg: _lib.Config) -> Tuple[torch.Tensor, Param
 
 
 def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
-               params_prev: Params, cfg: _lib.Config) -> Tuple[torch.Tensor, Params, int]:
+               params_prev: Params, cfg: _lib.Config,
+               ctx: Optional[Context] = None) -> Tuple[torch.Tensor, Params, int]:
     """One propagated frame.  ``params_prev['prior_counts']`` and
     ``params_prev['invalid_counts']`` are updated in place, as the reference's
     shift_and_fill does.  Returns (spix, params, adaptive_niters)."""
@@ -167,7 +172,10 @@ def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
     if spix_prev.shape != (H, W) or spix_prev.dtype != torch.int32:
         raise RuntimeError("spix_prev must be int32 (H, W)")
     nprev = int(params_prev["ids"].shape[0])
-    ctx = get_context(H, W, cfg, img.device)
+    if ctx is None:
+        ctx = get_context(H, W, cfg, img.device)
+    else:
+        ctx.set_config(cfg)
     cap = _lib.lib().stsp_prop_capacity(nprev)
     out = alloc_params(cap, img.device)
     spix = torch.empty((H, W), dtype=torch.int32, device=img.device)
@@ -199,3 +207,69 @@ def marked_video(vid: torch.Tensor, spix: torch.Tensor, color) -> torch.Tensor:
                                           marked.data_ptr(), _stream())
     _lib.check(rc, "stsp_marked_video")
     return marked
+
+
+class BatchRunner:
+    """Refines several propagated frames (one per Context) in shared launches
+    (stsp_prop_batch), optionally replaying a captured hipGraph."""
+
+    def __init__(self, max_frames: int = 16, use_graph: bool = False):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().stsp_batch_create(int(max_frames), ctypes.byref(h)),
+                   "stsp_batch_create")
+        self.handle = h
+        self.max_frames = int(max_frames)
+        _lib.check(_lib.lib().stsp_batch_use_graph(h, int(use_graph)), "stsp_batch_use_graph")
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().stsp_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prop(self, items, stream=None):
+        """items: list of (ctx, img, flow, spix_prev, params_prev).  Returns a
+        list of (spix, params, adaptive_niters) in the same order."""
+        n = len(items)
+        if n == 0:
+            return []
+        if n > self.max_frames:
+            raise RuntimeError(f"batch of {n} exceeds max_frames={self.max_frames}")
+        VP = ctypes.c_void_p
+        ctxs = (VP * n)()
+        imgs = (VP * n)()
+        flows = (VP * n)()
+        sprev = (VP * n)()
+        souts = (VP * n)()
+        prevs = (_lib.ParamsStruct * n)()
+        outs = (_lib.ParamsStruct * n)()
+        nprev = (ctypes.c_int * n)()
+        nout = (ctypes.c_int * n)()
+        niters = (ctypes.c_int * n)()
+        keep = []
+        for i, (ctx, img, flow, spix_prev, params_prev) in enumerate(items):
+            for t, name in ((img, "img"), (flow, "flow"), (spix_prev, "spix_prev")):
+                _check_input(t, name)
+            np_ = int(params_prev["ids"].shape[0])
+            cap = _lib.lib().stsp_prop_capacity(np_)
+            out = alloc_params(cap, img.device)
+            spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
+            ctxs[i] = ctx.handle.value
+            imgs[i] = img.data_ptr()
+            flows[i] = flow.data_ptr()
+            sprev[i] = spix_prev.data_ptr()
+            souts[i] = spix.data_ptr()
+            prevs[i] = params_struct(params_prev, np_)
+            outs[i] = params_struct(out, cap)
+            nprev[i] = np_
+            keep.append((spix, out))
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = _lib.lib().stsp_prop_batch(self.handle, ctxs, n, imgs, flows, sprev, prevs, nprev,
+                                        souts, outs, nout, niters, ctypes.c_void_p(s.cuda_stream))
+        _lib.check(rc, "stsp_prop_batch")
+        return [(keep[i][0], trim(keep[i][1], nout[i]), niters[i]) for i in range(n)]

@@ -1,0 +1,82 @@
+// Frame descriptors for batched (multi-frame) launches.
+//
+// The refinement loop of BASS / BIST (parameter update, split/merge, relabel,
+// boundary sweeps) runs as launches that cover every frame of a batch: a
+// device table holds one FrameDesc per frame, and each launch maps its block
+// index to a frame through prefix offsets (blocks never straddle frames, so
+// wave-level run aggregation stays within one frame).  A single frame is a
+// batch of one.  Per-label work of uneven size (the split BFS, the relabel
+// search) is pulled by persistent workgroups from compact work lists.
+#pragma once
+
+#include "kernels.h"
+
+namespace stsp {
+
+struct FrameDesc {
+  const float* img;
+  int* seg[2];            // ping-pong label maps; `cur` selects the live one
+  int* halves;
+  uint8_t* border;
+  const int* shifted;     // propagated frames: shifted labels (split ninvalid)
+  int* d1;
+  int* d2;
+  int H, W, npix;
+  Params P;
+  LabelStats st, sst;
+  SmState sm;
+  SweepRec* rec;
+  int nbuf, nsm, cap;
+  int* max_spix;
+  int* total;
+  int* counter;           // per-frame list counters (dead list length)
+  // relabel
+  const float* prev_prior_mu_app;
+  int nspix_prev;
+  int* alive;
+  int* target;
+  int* want_new;
+  int* rank;
+  int* src_of;
+  int* dead;              // dead previous ids
+  // settings
+  float sigma2_app, potts, split_alpha, merge_alpha, thresh_relabel, thresh_new;
+  int sp_size, prop_mode;
+  NigConsts nc;
+};
+
+// prefix offsets of one batch (device), each nf+1 long
+struct BatchOff {
+  int* pix;      // 256-pixel blocks
+  int* lab;      // 256-label blocks over cap
+  int* sm;       // 256-slot blocks over 2*cap+2
+  int* tile[3];  // sweep tiles for each launch shape
+};
+
+struct Batch {
+  FrameDesc* fr;  // device table
+  BatchOff off;
+  int nf;
+  int pix_blocks, lab_blocks, sm_blocks, tile_blocks[3];
+  int* work;      // work list (frame << 24 | label), capacity work_cap
+  int* work_n;    // [0] length, [1] cursor
+  int work_cap;
+};
+
+// host-side block counts for one frame
+inline int tiles_for_shape(int shape, int H, int W) {
+  static const int tw[3] = {32, 32, 64}, th[3] = {16, 32, 32};
+  return ((W + tw[shape] - 1) / tw[shape]) * ((H + th[shape] - 1) / th[shape]);
+}
+
+// refinement-loop launches over a batch (k_batch.hip)
+void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s);
+void b_param_summary(const Batch& b, int cur, hipStream_t s);
+void b_store_sample(const Batch& b, hipStream_t s);
+int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut& lut,
+             hipStream_t s);
+void b_split(const Batch& b, int cur, int direction, hipStream_t s);
+void b_merge(const Batch& b, int cur, int direction, hipStream_t s);
+void b_relabel(const Batch& b, int cur, hipStream_t s);
+
+}  // namespace stsp

@@ -1,24 +1,31 @@
-// Host orchestration of the BASS/BIST frame loop and the C ABI (stspix.h).
+// Host orchestration of BASS/BIST frames and the C ABI (stspix.h).
 //
-// One stsp_ctx owns the device workspace for a frame size.  A frame runs as
-// a sequence of kernel launches on the caller's stream; the only host waits
-// are the ones that size later launches (label counts after the first
-// compaction, the hole-filling loop and the final superpixel count).
+// A frame runs in three parts on the caller's stream:
+//   prologue   per frame: hexagonal init (BASS) or flow shift, hole filling
+//              and connectivity (propagated frames), parameter set-up;
+//   refinement one batched launch sequence for all frames of a batch
+//              (k_batch.hip): parameter updates, split/merge, relabel and the
+//              fused boundary sweeps, optionally replayed from a hipGraph;
+//   epilogue   per frame: compaction and parameter export.
+// Host waits happen only where later launch sizes need a device value: the
+// label count after the first compaction, the hole-filling loop and the
+// final superpixel count.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 #include "../../include/stspix.h"
-#include "kernels.h"
+#include "batch.h"
 
 using namespace stsp;
 
-#define HIP_TRY(x)                                   \
-  do {                                               \
-    hipError_t e_ = (x);                             \
-    if (e_ != hipSuccess) return STSP_ERR_HIP;       \
+#define HIP_TRY(x)                             \
+  do {                                         \
+    hipError_t e_ = (x);                       \
+    if (e_ != hipSuccess) return STSP_ERR_HIP; \
   } while (0)
 
 namespace {
@@ -75,10 +82,8 @@ struct Arena {
   }
 };
 
-}  // namespace
-
-// Event-pair timer for one kernel class (measures device time of the
-// launches it brackets on the launch stream).
+// Event-pair timer for one kernel class (device time of the launches it
+// brackets on the launch stream).
 struct KTimer {
   static const int kMax = 1024;
   hipEvent_t ev[2 * kMax];
@@ -87,17 +92,35 @@ struct KTimer {
   long long count = 0;
 };
 
+}  // namespace
+
+// A batch table on the device plus the cached refinement graph.
+struct stsp_batch {
+  int maxf = 0;
+  FrameDesc* d_fr = nullptr;
+  int* d_off = nullptr;  // 6 arrays of maxf+1
+  FrameDesc* h_fr = nullptr;
+  int* h_off = nullptr;
+  int* d_work = nullptr;
+  int* d_work_n = nullptr;
+  int work_cap = 0;
+  Batch b;
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;
+  int gcur = 0;
+};
+
 struct stsp_ctx {
   stsp_config cfg;
+  int H, W, npix;
+  int cap = 0, nprev = 0;
+  int sweep_shape = 2;  // 0: 1 sweep/launch, 1: 2, 2: 4
   int timing = 0;
   KTimer timers[STSP_KCLASS_COUNT];
-  int H, W, npix;
-  int cap = 0;    // label slots of the working parameter set
-  int nprev = 0;  // label slots of the previous-frame arrays
   SimpleLut lut;
   // pixel buffers
-  int *seg, *seg2, *halves, *d1, *d2, *list, *shifted, *parent, *rsize;
-  int sweep_shape = 2;  // 0: 1 sweep/launch, 1: 2, 2: 4 (fused)
+  int *seg, *seg2, *halves, *d1, *d2, *shifted, *parent, *rsize;
   uint8_t* border;
   unsigned long long* keys;
   // label buffers
@@ -105,7 +128,7 @@ struct stsp_ctx {
   LabelStats st, sst;
   SmState sm;
   SweepRec* rec;
-  int *off, *cursor, *alive, *flags, *rank, *target, *want_new, *src_of;
+  int *alive, *flags, *rank, *target, *want_new, *src_of, *dead;
   void* ctmp;
   // previous-frame buffers
   int64_t* fsum;
@@ -114,17 +137,25 @@ struct stsp_ctx {
   double* centers;
   unsigned long long* best;
   // scalars
-  int *max_spix, *total, *num_new, *num_neg;
+  int *max_spix, *total, *num_new, *num_neg, *counter;
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
   void* prev_mem = nullptr;
+  // per-frame state between prologue and epilogue
+  int nbuf = 0, nspix_prev = 0, prop_mode = 0, niters = 0;
+  const float* img = nullptr;
+  const float* prev_prior_mu_app = nullptr;
+  stsp_batch self;  // batch of one for single-frame calls
 };
 
+// ---------------------------------------------------------------------------
+// allocation
+// ---------------------------------------------------------------------------
 static int alloc_pixels(stsp_ctx* c) {
   size_t n = c->npix;
   Arena a;
-  a.size = 13 * 256 + n * (4 * 9 + 1 + 8) + 4096;
+  a.size = 16 * 256 + n * (4 * 8 + 1 + 8) + 4096;
   HIP_TRY(hipMalloc(&c->pix_mem, a.size));
   a.base = (char*)c->pix_mem;
   c->seg = a.take<int>(n);
@@ -132,7 +163,6 @@ static int alloc_pixels(stsp_ctx* c) {
   c->halves = a.take<int>(n);
   c->d1 = a.take<int>(n);
   c->d2 = a.take<int>(n);
-  c->list = a.take<int>(n);
   c->shifted = a.take<int>(n);
   c->parent = a.take<int>(n);
   c->rsize = a.take<int>(n);
@@ -142,8 +172,17 @@ static int alloc_pixels(stsp_ctx* c) {
   c->total = a.take<int>(1);
   c->num_new = a.take<int>(1);
   c->num_neg = a.take<int>(1);
+  c->counter = a.take<int>(4);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
+}
+
+static void invalidate_graph(stsp_batch* bh) {
+  if (bh->gexec) {
+    (void)hipGraphExecDestroy(bh->gexec);
+    bh->gexec = nullptr;
+  }
+  bh->gkey.clear();
 }
 
 // (re)allocate the per-label workspace for `cap` slots
@@ -156,12 +195,10 @@ static int ensure_labels(stsp_ctx* c, int cap) {
   }
   cap = cap + cap / 4;  // headroom for growth across frames
   size_t n = cap, nsm = 2 * n + 2;
-  // byte budget, generously rounded (each take pads to 256 B)
-  size_t bytes = n * (3 * 4 * 2 + 8 * (2 + 3 + 2 + 3 + 3 + 3 + 1) + 4 * 6) +  // params
-                 n * (4 + 8 * 11 + 4 + 16) +                                   // st
-                 nsm * (4 + 8 * 11 + 4) +                                      // sst
-                 nsm * (4 * 4 + 8 * 3 * 2 + 8 * 2 + 4 * 8) + nsm * 8 +          // sm
-                 n * sizeof(SweepRec) + n * 4 * 8 + compact_tmp_bytes(cap) + 96 * 256;
+  size_t bytes = n * (3 * 4 * 2 + 8 * (2 + 3 + 2 + 3 + 3 + 3 + 1) + 4 * 6) +
+                 n * (4 + 8 * 11 + 4 + 16) + nsm * (4 + 8 * 11 + 4) +
+                 nsm * (4 * 4 + 8 * 3 * 2 + 8 * 2 + 4 * 8) + nsm * 8 + n * sizeof(SweepRec) +
+                 n * 4 * 8 + compact_tmp_bytes(cap) + 100 * 256;
   HIP_TRY(hipMalloc(&c->lab_mem, bytes));
   Arena a;
   a.base = (char*)c->lab_mem;
@@ -215,20 +252,20 @@ static int ensure_labels(stsp_ctx* c, int cap) {
   sm.valid0 = a.take<int>(nsm);
   sm.pc0 = a.take<float>(nsm);
   c->rec = a.take<SweepRec>(n);
-  c->off = a.take<int>(n);
-  c->cursor = a.take<int>(n);
   c->alive = a.take<int>(n);
   c->flags = a.take<int>(n);
   c->rank = a.take<int>(n);
   c->target = a.take<int>(n);
   c->want_new = a.take<int>(n);
   c->src_of = a.take<int>(n);
+  c->dead = a.take<int>(n);
   c->ctmp = (void*)a.take<char>(compact_tmp_bytes(cap));
   if (a.used > a.size) {
     fprintf(stderr, "stspix: label arena overflow (%zu > %zu)\n", a.used, a.size);
     return STSP_ERR_HIP;
   }
   c->cap = cap;
+  invalidate_graph(&c->self);
   return STSP_OK;
 }
 
@@ -256,12 +293,38 @@ static int ensure_prev(stsp_ctx* c, int n) {
   return STSP_OK;
 }
 
-__global__ void k_set_int(int* p, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+static int batch_alloc(stsp_batch* bh, int maxf) {
+  bh->maxf = maxf;
+  HIP_TRY(hipMalloc((void**)&bh->d_fr, sizeof(FrameDesc) * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_off, sizeof(int) * 6 * (maxf + 1)));
+  HIP_TRY(hipMalloc((void**)&bh->d_work_n, sizeof(int) * 4));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_fr, sizeof(FrameDesc) * maxf, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_off, sizeof(int) * 6 * (maxf + 1), hipHostMallocDefault));
+  return STSP_OK;
 }
 
+static void batch_free(stsp_batch* bh) {
+  invalidate_graph(bh);
+  if (bh->d_fr) (void)hipFree(bh->d_fr);
+  if (bh->d_off) (void)hipFree(bh->d_off);
+  if (bh->d_work) (void)hipFree(bh->d_work);
+  if (bh->d_work_n) (void)hipFree(bh->d_work_n);
+  if (bh->h_fr) (void)hipHostFree(bh->h_fr);
+  if (bh->h_off) (void)hipHostFree(bh->h_off);
+  bh->d_fr = nullptr;
+  bh->d_off = nullptr;
+  bh->d_work = nullptr;
+  bh->d_work_n = nullptr;
+  bh->h_fr = nullptr;
+  bh->h_off = nullptr;
+  bh->work_cap = 0;
+}
+
+// ---------------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------------
 static void tstart(stsp_ctx* c, int cls, hipStream_t s) {
-  if (!c->timing) return;
+  if (!c || !c->timing) return;
   KTimer& t = c->timers[cls];
   if (t.n >= KTimer::kMax) return;
   if (t.created <= t.n) {
@@ -272,13 +335,12 @@ static void tstart(stsp_ctx* c, int cls, hipStream_t s) {
   (void)hipEventRecord(t.ev[2 * t.n], s);
 }
 static void tstop(stsp_ctx* c, int cls, hipStream_t s) {
-  if (!c->timing) return;
+  if (!c || !c->timing) return;
   KTimer& t = c->timers[cls];
   if (t.n >= KTimer::kMax) return;
   (void)hipEventRecord(t.ev[2 * t.n + 1], s);
   t.n++;
 }
-// called after the stream has been synchronised
 static void tflush(stsp_ctx* c) {
   if (!c->timing) return;
   for (int k = 0; k < STSP_KCLASS_COUNT; k++) {
@@ -294,6 +356,10 @@ static void tflush(stsp_ctx* c) {
   }
 }
 
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 static int read_int(stsp_ctx* c, const int* dev, hipStream_t s, int* out) {
   HIP_TRY(hipMemcpyAsync(c->host, dev, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -301,33 +367,102 @@ static int read_int(stsp_ctx* c, const int* dev, hipStream_t s, int* out) {
   return STSP_OK;
 }
 
-static void update_params(stsp_ctx* c, const float* img, int nbuf, hipStream_t s,
-                          bool bbox = false) {
-  tstart(c, STSP_KCLASS_STATS, s);
-  launch_stats(img, c->seg, nullptr, c->npix, c->W, c->st, nbuf, s, bbox);
-  tstop(c, STSP_KCLASS_STATS, s);
-  launch_param_update(c->P, c->st, c->rec, nbuf, c->cfg.sp_size, 0, s);
+// ---------------------------------------------------------------------------
+// batch tables
+// ---------------------------------------------------------------------------
+static FrameDesc make_desc(stsp_ctx* c) {
+  FrameDesc d;
+  memset(&d, 0, sizeof(d));
+  d.img = c->img;
+  d.seg[0] = c->seg;
+  d.seg[1] = c->seg2;
+  d.halves = c->halves;
+  d.border = c->border;
+  d.shifted = c->prop_mode ? c->shifted : nullptr;
+  d.d1 = c->d1;
+  d.d2 = c->d2;
+  d.H = c->H;
+  d.W = c->W;
+  d.npix = c->npix;
+  d.P = c->P;
+  d.st = c->st;
+  d.sst = c->sst;
+  d.sm = c->sm;
+  d.rec = c->rec;
+  d.nbuf = c->nbuf;
+  d.nsm = 2 * c->nbuf + 2;
+  d.cap = c->cap;
+  d.max_spix = c->max_spix;
+  d.total = c->total;
+  d.counter = c->counter;
+  d.prev_prior_mu_app = c->prev_prior_mu_app;
+  d.nspix_prev = c->nspix_prev;
+  d.alive = c->alive;
+  d.target = c->target;
+  d.want_new = c->want_new;
+  d.rank = c->rank;
+  d.src_of = c->src_of;
+  d.dead = c->dead;
+  d.sigma2_app = c->cfg.sigma2_app;
+  d.potts = c->cfg.potts;
+  d.split_alpha = c->cfg.split_alpha;
+  d.merge_alpha = c->cfg.merge_alpha;
+  d.thresh_relabel = c->cfg.thresh_relabel;
+  d.thresh_new = c->cfg.thresh_new;
+  d.sp_size = c->cfg.sp_size;
+  d.prop_mode = c->prop_mode;
+  d.nc = make_nig(c->cfg);
+  return d;
 }
 
-static void split_step(stsp_ctx* c, const float* img, const int* shifted, int nbuf,
-                       int direction, const NigConsts& nc, int prop_mode, hipStream_t s) {
-  int nsm = 2 * nbuf + 2;
-  launch_sm_clear(c->sm, c->sst, nsm, nbuf, s);
-  launch_split_proposal(c->seg, c->P, c->st.bbox, nbuf, c->max_spix, direction, c->W, c->H,
-                        c->d1, c->d2, c->halves, s);
-  launch_stats(img, c->halves, shifted, c->npix, c->W, c->sst, nsm, s);
-  launch_split_eval_apply(c->seg, c->halves, c->P, c->st, c->sst, c->sm, nbuf, c->max_spix, nc,
-                          c->cfg.split_alpha, prop_mode, c->npix, c->total, s);
-}
-
-// niters_seg boundary sweeps; the result lands in c->seg
-static void sweeps(stsp_ctx* c, const float* img, hipStream_t s) {
-  int* res = launch_sweeps(img, c->seg, c->seg2, c->rec, c->cfg.niters_seg, c->cfg.sigma2_app,
-                           c->cfg.potts, c->H, c->W, c->lut, c->sweep_shape, s);
-  if (res != c->seg) {
-    c->seg2 = c->seg;
-    c->seg = res;
+// Fills the device table for frames ctxs[0..n); grid sizes cover each
+// frame's capacity so that a captured graph stays valid across frames.
+static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
+  if (n > bh->maxf || n > 64) return STSP_ERR_ARG;
+  const int M = bh->maxf + 1;
+  int* off = bh->h_off;
+  int* o_pix = off;
+  int* o_lab = off + M;
+  int* o_sm = off + 2 * M;
+  int* o_t[3] = {off + 3 * M, off + 4 * M, off + 5 * M};
+  int work = 0;
+  o_pix[0] = o_lab[0] = o_sm[0] = o_t[0][0] = o_t[1][0] = o_t[2][0] = 0;
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = ctxs[i];
+    bh->h_fr[i] = make_desc(c);
+    o_pix[i + 1] = o_pix[i] + div_up(c->npix, kBlock);
+    o_lab[i + 1] = o_lab[i] + div_up(c->cap, kBlock);
+    o_sm[i + 1] = o_sm[i] + div_up(2 * c->cap + 2, kBlock);
+    for (int sh = 0; sh < 3; sh++) o_t[sh][i + 1] = o_t[sh][i] + tiles_for_shape(sh, c->H, c->W);
+    work += c->cap;
   }
+  if (work > bh->work_cap) {
+    if (bh->d_work) {
+      HIP_TRY(hipStreamSynchronize(s));
+      HIP_TRY(hipFree(bh->d_work));
+    }
+    HIP_TRY(hipMalloc((void**)&bh->d_work, sizeof(int) * (size_t)work));
+    bh->work_cap = work;
+    invalidate_graph(bh);
+  }
+  HIP_TRY(hipMemcpyAsync(bh->d_fr, bh->h_fr, sizeof(FrameDesc) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(bh->d_off, bh->h_off, sizeof(int) * 6 * M, hipMemcpyHostToDevice, s));
+  Batch& b = bh->b;
+  b.fr = bh->d_fr;
+  int* d = bh->d_off;
+  b.off.pix = d;
+  b.off.lab = d + M;
+  b.off.sm = d + 2 * M;
+  for (int sh = 0; sh < 3; sh++) b.off.tile[sh] = d + (3 + sh) * M;
+  b.nf = n;
+  b.pix_blocks = o_pix[n];
+  b.lab_blocks = o_lab[n];
+  b.sm_blocks = o_sm[n];
+  for (int sh = 0; sh < 3; sh++) b.tile_blocks[sh] = o_t[sh][n];
+  b.work = bh->d_work;
+  b.work_n = bh->d_work_n;
+  b.work_cap = bh->work_cap;
+  return STSP_OK;
 }
 
 static inline bool is_split_iter(const stsp_config& cfg, int idx) {
@@ -339,31 +474,121 @@ static inline bool is_merge_iter(const stsp_config& cfg, int idx) {
   return idx > cfg.sm_start && idx % p == p / 2;
 }
 
-static int finish_frame(stsp_ctx* c, int nbuf, int prev_nspix, int* spix_out, stsp_params* out,
-                        int* nspix_out, hipStream_t s) {
-  const float* dummy = nullptr;
-  (void)dummy;
-  launch_store_sample(c->P, c->st, nbuf, s);
-  launch_compact(c->seg, c->npix, &c->P, nbuf, prev_nspix, nbuf, c->alive, c->flags, c->rank,
-                 c->num_new, c->ctmp, s);
-  int num_new = 0;
-  int rc = read_int(c, c->num_new, s, &num_new);
-  if (rc) return rc;
-  int nspix = prev_nspix + num_new;
-  *nspix_out = nspix;
-  if (nspix > out->cap) return STSP_ERR_CAPACITY;
-  k_set_int<<<1, 1, 0, s>>>(c->total, nspix);
-  launch_fill(c->alive, 0, nbuf, s);
-  launch_mark_alive(c->seg, c->npix, c->alive, nbuf, s);
-  OutParams o;
-  memcpy(&o, out, sizeof(o));
-  launch_export(c->P, o, c->total, c->alive, nspix, s);
-  HIP_TRY(hipMemcpyAsync(spix_out, c->seg, sizeof(int) * c->npix, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipGetLastError());
-  if (c->timing) {
-    HIP_TRY(hipStreamSynchronize(s));
-    tflush(c);
+// The refinement loop of bass() (bass.cu:71-168) / prop_bass_v3
+// (prop.cu:364-457) for every frame of the batch; returns the live buffer
+// index.  `lead` supplies the settings shared by the batch and the timers.
+static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_mode,
+                           int shape, hipStream_t s) {
+  const stsp_config& cfg = lead->cfg;
+  int cur = 0;
+  int smc = 1;  // split/merge counter (bass.cu:56, prop.cu:343)
+  for (int idx = 0; idx < niters; idx++) {
+    tstart(lead, STSP_KCLASS_STATS, s);
+    b_update_params(b, cur, is_split_iter(cfg, idx), s);
+    tstop(lead, STSP_KCLASS_STATS, s);
+    if (is_split_iter(cfg, idx)) {
+      smc += 1;
+      b_split(b, cur, smc % 2 + 1, s);
+      b_update_params(b, cur, false, s);
+    }
+    if (is_merge_iter(cfg, idx)) {
+      if (prop_mode) {
+        b_relabel(b, cur, s);
+        smc += 1;
+      }
+      b_merge(b, cur, smc % 2, s);
+      b_update_params(b, cur, false, s);
+    }
+    tstart(lead, STSP_KCLASS_SWEEP, s);
+    cur = b_sweeps(b, cur, cfg.niters_seg, shape, lead->lut, s);
+    tstop(lead, STSP_KCLASS_SWEEP, s);
   }
+  b_update_params(b, cur, false, s);
+  b_store_sample(b, s);
+  return cur;
+}
+
+// Runs the refinement of frames ctxs[0..n) (prologues done) as one batch.
+static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
+  stsp_ctx* lead = ctxs[0];
+  const int niters = lead->niters, prop_mode = lead->prop_mode, shape = lead->sweep_shape;
+  for (int i = 1; i < n; i++) {
+    const stsp_config& a = ctxs[i]->cfg;
+    const stsp_config& l = lead->cfg;
+    if (ctxs[i]->niters != niters || ctxs[i]->prop_mode != prop_mode ||
+        a.niters_seg != l.niters_seg || a.sm_period != l.sm_period || a.sm_start != l.sm_start)
+      return STSP_ERR_ARG;
+  }
+  int rc = batch_build(bh, ctxs, n, s);
+  if (rc) return rc;
+  int cur;
+  bool timing = lead->timing != 0;
+  if (bh->use_graph && !timing) {
+    std::vector<long long> key = {n, niters, prop_mode, shape, lead->cfg.niters_seg,
+                                  lead->cfg.sm_period, lead->cfg.sm_start, bh->work_cap};
+    for (int i = 0; i < n; i++) {
+      key.push_back((long long)(size_t)ctxs[i]);
+      key.push_back(ctxs[i]->cap);
+    }
+    if (!bh->gexec || key != bh->gkey) {
+      invalidate_graph(bh);
+      hipGraph_t g;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int c0 = refine_launches(lead, bh->b, niters, prop_mode, shape, s);
+      HIP_TRY(hipStreamEndCapture(s, &g));
+      HIP_TRY(hipGraphInstantiate(&bh->gexec, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      bh->gkey = key;
+      bh->gcur = c0;
+    }
+    HIP_TRY(hipGraphLaunch(bh->gexec, s));
+    cur = bh->gcur;
+  } else {
+    cur = refine_launches(lead, bh->b, niters, prop_mode, shape, s);
+  }
+  for (int i = 0; i < n; i++) {
+    if (cur == 1) {
+      int* t = ctxs[i]->seg;
+      ctxs[i]->seg = ctxs[i]->seg2;
+      ctxs[i]->seg2 = t;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// prologues and epilogue
+// ---------------------------------------------------------------------------
+static int bass_prologue(stsp_ctx* c, const float* img, hipStream_t s) {
+  const stsp_config& cfg = c->cfg;
+  int H = c->H, W = c->W, npix = c->npix;
+  HexGeom g = hex_geometry(cfg.sp_size, W, H);
+  int K0 = g.nx * g.ny;
+  int nbuf = 30 * K0;  // bass.cu:245
+  int rc = ensure_labels(c, nbuf);
+  if (rc) return rc;
+  launch_hex_init(c->seg, H, W, cfg.sp_size, s);
+  launch_compact(c->seg, npix, nullptr, K0, 0, nbuf, c->alive, c->flags, c->rank, c->num_new,
+                 c->ctmp, s);
+  int nspix = 0;
+  rc = read_int(c, c->num_new, s, &nspix);
+  if (rc) return rc;
+  c->img = img;
+  c->nbuf = nbuf;
+  c->nspix_prev = 0;
+  c->prop_mode = 0;
+  c->niters = cfg.niters;
+  c->prev_prior_mu_app = nullptr;
+  // mark_active_contiguous + init_sp_params (init_sparams.cu:25-231)
+  launch_bass_mark(c->P, nspix, nbuf, cfg.sp_size, s);
+  stsp_ctx* one = c;
+  rc = batch_build(&c->self, &one, 1, s);
+  if (rc) return rc;
+  b_param_summary(c->self.b, 0, s);
+  launch_bass_init(c->P, nspix, nbuf, cfg.sp_size, npix, s);
+  k_set_int<<<1, 1, 0, s>>>(c->max_spix, nspix - 1);
+  HIP_TRY(hipGetLastError());
   return STSP_OK;
 }
 
@@ -379,6 +604,94 @@ static int fill_loop(stsp_ctx* c, hipStream_t s) {
     if (iter > 0 && neg == prev_neg) return STSP_ERR_FILL;
     launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, s);
     prev_neg = neg;
+  }
+  return STSP_OK;
+}
+
+// shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up (prop.cu:677-745)
+static int prop_prologue(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
+                         stsp_params* prev, int n, int* niters_state, hipStream_t s) {
+  const stsp_config& cfg = c->cfg;
+  int H = c->H, W = c->W, npix = c->npix;
+  int nbuf = 10 * n;  // prop.cu:702
+  int rc = ensure_labels(c, nbuf);
+  if (rc) return rc;
+  rc = ensure_prev(c, n);
+  if (rc) return rc;
+  launch_flow_pool(flow, spix_prev, npix, c->fsum, c->counts_prev, prev->mu_shape, c->flow_ds,
+                   c->centers, n, s);
+  launch_shift(spix_prev, c->flow_ds, c->counts_prev, c->keys, c->shifted, W, H, s);
+  HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * npix, hipMemcpyDeviceToDevice, s));
+  launch_count_labels(c->shifted, npix, c->counts_s, n, s);
+  rc = fill_loop(c, s);
+  if (rc) return rc;
+  launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, W, H, n, s);
+  rc = fill_loop(c, s);
+  if (rc) return rc;
+  launch_count_labels(c->seg, npix, c->counts_f, n, s);
+  launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prev->prior_counts,
+                      prev->invalid_counts, n, s);
+  // adaptive iteration count (pyapi.cu:139-149)
+  k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
+  launch_fill_count(c->shifted, c->border, c->num_neg, W, H, s);
+  int ninvalid = 0;
+  rc = read_int(c, c->num_neg, s, &ninvalid);
+  if (rc) return rc;
+  float iperc = (float)(ninvalid / (1.0 * npix));
+  int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
+  if (niters_state) *niters_state = adapt;
+  InParams in;
+  memcpy(&in, prev, sizeof(in));
+  launch_prop_params_init(c->P, in, n, nbuf, cfg.sp_size, c->seg, npix, s);
+  k_set_int<<<1, 1, 0, s>>>(c->max_spix, n - 1);
+  c->img = img;
+  c->nbuf = nbuf;
+  c->nspix_prev = n;
+  c->prop_mode = 1;
+  c->niters = cfg.prop_niters > 0 ? cfg.prop_niters : adapt;
+  c->prev_prior_mu_app = prev->prior_mu_app;
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
+// compaction (compact_spix.cu:41-122 as called at prop.cu:820-829 and
+// bass.cu:319-320); the new-label count is copied to pinned memory
+static int epilogue_compact(stsp_ctx* c, hipStream_t s) {
+  launch_compact(c->seg, c->npix, &c->P, c->nbuf, c->nspix_prev, c->nbuf, c->alive, c->flags,
+                 c->rank, c->num_new, c->ctmp, s);
+  HIP_TRY(hipMemcpyAsync(c->host + 1, c->num_new, sizeof(int), hipMemcpyDeviceToHost, s));
+  return STSP_OK;
+}
+
+// export after the stream has been synchronised (sparams_io.cu:65-79, 583-688)
+static int epilogue_finish(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_out,
+                           hipStream_t s) {
+  int nspix = c->nspix_prev + c->host[1];
+  *nspix_out = nspix;
+  if (nspix > out->cap) return STSP_ERR_CAPACITY;
+  k_set_int<<<1, 1, 0, s>>>(c->total, nspix);
+  launch_fill(c->alive, 0, c->nbuf, s);
+  launch_mark_alive(c->seg, c->npix, c->alive, c->nbuf, s);
+  OutParams o;
+  memcpy(&o, out, sizeof(o));
+  launch_export(c->P, o, c->total, c->alive, nspix, s);
+  HIP_TRY(hipMemcpyAsync(spix_out, c->seg, sizeof(int) * c->npix, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
+static int run_single(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_out,
+                      hipStream_t s) {
+  stsp_ctx* one = c;
+  int rc = refine(&c->self, &one, 1, s);
+  if (!rc) rc = epilogue_compact(c, s);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  rc = epilogue_finish(c, spix_out, out, nspix_out, s);
+  if (rc) return rc;
+  if (c->timing) {
+    HIP_TRY(hipStreamSynchronize(s));
+    tflush(c);
   }
   return STSP_OK;
 }
@@ -431,6 +744,7 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
   c->npix = height * width;
   c->lut = make_lut();
   int rc = alloc_pixels(c);
+  if (!rc) rc = batch_alloc(&c->self, 1);
   if (rc) {
     stsp_ctx_destroy(c);
     return rc;
@@ -444,11 +758,24 @@ void stsp_ctx_destroy(stsp_ctx* c) {
   (void)hipDeviceSynchronize();
   for (int k = 0; k < STSP_KCLASS_COUNT; k++)
     for (int i = 0; i < 2 * c->timers[k].created; i++) (void)hipEventDestroy(c->timers[k].ev[i]);
+  batch_free(&c->self);
   if (c->pix_mem) (void)hipFree(c->pix_mem);
   if (c->lab_mem) (void)hipFree(c->lab_mem);
   if (c->prev_mem) (void)hipFree(c->prev_mem);
   if (c->host) (void)hipHostFree(c->host);
   delete c;
+}
+
+int stsp_ctx_set_config(stsp_ctx* c, const stsp_config* cfg) {
+  if (!c || !cfg || cfg->sp_size < 1) return STSP_ERR_ARG;
+  c->cfg = *cfg;
+  return STSP_OK;
+}
+
+int stsp_ctx_set_tuning(stsp_ctx* c, int sweep_shape) {
+  if (!c || sweep_shape < 0 || sweep_shape > 2) return STSP_ERR_ARG;
+  c->sweep_shape = sweep_shape;
+  return STSP_OK;
 }
 
 int stsp_ctx_enable_timing(stsp_ctx* c, int on) {
@@ -470,58 +797,12 @@ int stsp_ctx_kernel_time(stsp_ctx* c, int kclass, double* total_ms, long long* l
   return STSP_OK;
 }
 
-int stsp_ctx_set_tuning(stsp_ctx* c, int sweep_shape) {
-  if (!c || sweep_shape < 0 || sweep_shape > 2) return STSP_ERR_ARG;
-  c->sweep_shape = sweep_shape;
-  return STSP_OK;
-}
-
-int stsp_ctx_set_config(stsp_ctx* c, const stsp_config* cfg) {
-  if (!c || !cfg || cfg->sp_size < 1) return STSP_ERR_ARG;
-  c->cfg = *cfg;
-  return STSP_OK;
-}
-
 int stsp_bass(stsp_ctx* c, const float* img, int* spix_out, stsp_params* out, int* nspix_out,
               hipStream_t s) {
   if (!c || !img || !spix_out || !out || !nspix_out) return STSP_ERR_ARG;
-  const stsp_config& cfg = c->cfg;
-  int H = c->H, W = c->W, npix = c->npix;
-  HexGeom g = hex_geometry(cfg.sp_size, W, H);
-  int K0 = g.nx * g.ny;
-  int nbuf = 30 * K0;
-  int rc = ensure_labels(c, nbuf);
+  int rc = bass_prologue(c, img, s);
   if (rc) return rc;
-  launch_hex_init(c->seg, H, W, cfg.sp_size, s);
-  launch_compact(c->seg, npix, nullptr, K0, 0, nbuf, c->alive, c->flags, c->rank, c->num_new,
-                 c->ctmp, s);
-  int nspix = 0;
-  rc = read_int(c, c->num_new, s, &nspix);
-  if (rc) return rc;
-  launch_bass_params_init(c->P, nspix, nbuf, cfg.sp_size, npix, c->st, img, c->seg, W, s);
-  launch_border(c->seg, c->border, H, W, s);
-  k_set_int<<<1, 1, 0, s>>>(c->max_spix, nspix - 1);
-  NigConsts nc = make_nig(cfg);
-  int smc = 1;
-  for (int idx = 0; idx < cfg.niters; idx++) {
-    update_params(c, img, nbuf, s, is_split_iter(cfg, idx));
-    if (is_split_iter(cfg, idx)) {
-      smc += 1;
-      split_step(c, img, nullptr, nbuf, smc % 2 + 1, nc, 0, s);
-      update_params(c, img, nbuf, s);
-    }
-    if (is_merge_iter(cfg, idx)) {
-      launch_border(c->seg, c->border, H, W, s);
-      launch_merge(c->seg, c->border, c->P, c->st, c->sst, c->sm, nbuf, 2 * nbuf + 2, nc, 0.0f,
-                   smc % 2, 0, img, W, H, s);
-      update_params(c, img, nbuf, s);
-    }
-    tstart(c, STSP_KCLASS_SWEEP, s);
-    sweeps(c, img, s);
-    tstop(c, STSP_KCLASS_SWEEP, s);
-  }
-  update_params(c, img, nbuf, s);
-  return finish_frame(c, nbuf, 0, spix_out, out, nspix_out, s);
+  return run_single(c, spix_out, out, nspix_out, s);
 }
 
 int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
@@ -530,76 +811,87 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
   if (!c || !img || !flow || !spix_prev || !prev || !spix_out || !out || !nspix_out ||
       nspix_prev < 1)
     return STSP_ERR_ARG;
-  const stsp_config& cfg = c->cfg;
-  int H = c->H, W = c->W, npix = c->npix, n = nspix_prev;
-  int nbuf = 10 * n;
-  int rc = ensure_labels(c, nbuf);
+  int rc = prop_prologue(c, img, flow, spix_prev, prev, nspix_prev, niters_state, s);
   if (rc) return rc;
-  rc = ensure_prev(c, n);
-  if (rc) return rc;
-  // shift_and_fill (shift_and_fill.cu:108-339)
-  launch_flow_pool(flow, spix_prev, npix, c->fsum, c->counts_prev, prev->mu_shape, c->flow_ds,
-                   c->centers, n, s);
-  launch_shift(spix_prev, c->flow_ds, c->counts_prev, c->keys, c->shifted, W, H, s);
-  HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * npix, hipMemcpyDeviceToDevice, s));
-  launch_count_labels(c->shifted, npix, c->counts_s, n, s);
-  rc = fill_loop(c, s);
-  if (rc) return rc;
-  launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, W, H, n, s);
-  rc = fill_loop(c, s);
-  if (rc) return rc;
-  launch_count_labels(c->seg, npix, c->counts_f, n, s);
-  launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prev->prior_counts,
-                      prev->invalid_counts, n, s);
-  // adaptive iteration count (pyapi.cu:139-149)
-  k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
-  launch_fill_count(c->shifted, c->border, c->num_neg, W, H, s);
-  int ninvalid = 0;
-  rc = read_int(c, c->num_neg, s, &ninvalid);
-  if (rc) return rc;
-  float iperc = (float)(ninvalid / (1.0 * npix));
-  int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
-  if (niters_state) *niters_state = adapt;
-  int niters = cfg.prop_niters > 0 ? cfg.prop_niters : adapt;
-  // run_prop / prop_bass_v3 (prop.cu:326-468, 677-865)
-  InParams in;
-  memcpy(&in, prev, sizeof(in));
-  launch_prop_params_init(c->P, in, n, nbuf, cfg.sp_size, c->seg, npix, s);
-  k_set_int<<<1, 1, 0, s>>>(c->max_spix, n - 1);
-  NigConsts nc = make_nig(cfg);
-  int smc = 1;
-  for (int idx = 0; idx < niters; idx++) {
-    update_params(c, img, nbuf, s, is_split_iter(cfg, idx));
-    if (is_split_iter(cfg, idx)) {
-      smc += 1;
-      split_step(c, img, c->shifted, nbuf, smc % 2 + 1, nc, 1, s);
-      update_params(c, img, nbuf, s);
-    }
-    if (is_merge_iter(cfg, idx)) {
-      launch_relabel(c->seg, npix, c->P, nbuf, prev->prior_mu_app, n, c->max_spix,
-                     cfg.thresh_relabel, cfg.thresh_new, c->alive, c->target, c->want_new,
-                     c->rank, c->src_of, c->total, s);
-      smc += 1;
-      launch_border(c->seg, c->border, H, W, s);
-      launch_merge(c->seg, c->border, c->P, c->st, c->sst, c->sm, nbuf, 2 * nbuf + 2, nc,
-                   cfg.merge_alpha, smc % 2, 1, img, W, H, s);
-      update_params(c, img, nbuf, s);
-    }
-    tstart(c, STSP_KCLASS_SWEEP, s);
-    sweeps(c, img, s);
-    tstop(c, STSP_KCLASS_SWEEP, s);
-  }
-  update_params(c, img, nbuf, s);
-  return finish_frame(c, nbuf, n, spix_out, out, nspix_out, s);
+  return run_single(c, spix_out, out, nspix_out, s);
 }
 
-}  // extern "C"
+int stsp_batch_create(int max_frames, stsp_batch** out) {
+  if (!out || max_frames < 1 || max_frames > 64) return STSP_ERR_ARG;
+  stsp_batch* bh = new (std::nothrow) stsp_batch();
+  if (!bh) return STSP_ERR_HIP;
+  int rc = batch_alloc(bh, max_frames);
+  if (rc) {
+    stsp_batch_destroy(bh);
+    return rc;
+  }
+  *out = bh;
+  return STSP_OK;
+}
 
-extern "C" int stsp_marked_video(const float* vid, const int* spix, const float* color_host,
-                                 int nframes, int height, int width, float* marked,
-                                 hipStream_t stream) {
+void stsp_batch_destroy(stsp_batch* bh) {
+  if (!bh) return;
+  (void)hipDeviceSynchronize();
+  batch_free(bh);
+  delete bh;
+}
+
+int stsp_batch_use_graph(stsp_batch* bh, int on) {
+  if (!bh) return STSP_ERR_ARG;
+  bh->use_graph = on != 0;
+  if (!on) invalidate_graph(bh);
+  return STSP_OK;
+}
+
+int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
+                    const float** flows, const int** spix_prev, stsp_params* prevs,
+                    const int* nspix_prev, int** spix_out, stsp_params* outs, int* nspix_out,
+                    int* niters_state, hipStream_t s) {
+  if (!bh || !ctxs || n < 1 || n > bh->maxf || !imgs || !flows || !spix_prev || !prevs ||
+      !nspix_prev || !spix_out || !outs || !nspix_out)
+    return STSP_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || nspix_prev[i] < 1) return STSP_ERR_ARG;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return STSP_ERR_ARG;
+  }
+  for (int i = 0; i < n; i++) {
+    int rc = prop_prologue(ctxs[i], imgs[i], flows[i], spix_prev[i], &prevs[i], nspix_prev[i],
+                           niters_state ? &niters_state[i] : nullptr, s);
+    if (rc) return rc;
+  }
+  // frames with the same schedule refine together
+  std::vector<int> done(n, 0);
+  std::vector<stsp_ctx*> group;
+  for (int i = 0; i < n; i++) {
+    if (done[i]) continue;
+    group.clear();
+    for (int j = i; j < n; j++) {
+      if (done[j] || ctxs[j]->niters != ctxs[i]->niters) continue;
+      group.push_back(ctxs[j]);
+      done[j] = 1;
+    }
+    int rc = refine(bh, group.data(), (int)group.size(), s);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < n; i++) {
+    int rc = epilogue_compact(ctxs[i], s);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) {
+    int rc = epilogue_finish(ctxs[i], spix_out[i], &outs[i], &nspix_out[i], s);
+    if (rc) return rc;
+  }
+  return STSP_OK;
+}
+
+int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,
+                      int height, int width, float* marked, hipStream_t stream) {
   if (!vid || !spix || !color_host || !marked || nframes < 1 || height < 1 || width < 1)
     return STSP_ERR_ARG;
   launch_marked(vid, spix, color_host, nframes, height, width, marked, stream);
   return hipGetLastError() == hipSuccess ? STSP_OK : STSP_ERR_HIP;
 }
+
+}  // extern "C"

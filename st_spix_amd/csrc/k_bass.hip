@@ -1,7 +1,6 @@
-// Per-pixel and per-label kernels of the single-image BASS loop:
-// hexagonal initialisation, border detection, the checkerboard boundary
-// update, deterministic sufficient statistics, the Normal-Inverse-Wishart
-// parameter update, compaction and parameter export.
+// Per-frame kernels around the refinement loop: hexagonal initialisation,
+// first-frame parameter initialisation, scans, compaction, parameter export
+// and the border overlay.  The refinement loop itself is in k_batch.hip.
 #include "kernels.h"
 
 namespace stsp {
@@ -54,390 +53,6 @@ HexGeom hex_geometry(int sp_size, int width, int height) {
   g.nx = (int)floor((double)width / g.w) + 2;
   g.ny = (int)floor((double)height / (1.5 * g.Hh)) + 2;
   return g;
-}
-
-// ---------------------------------------------------------------------------
-// border detection (reference seg_utils.cu:164-195)
-// ---------------------------------------------------------------------------
-__global__ void k_border(const int* __restrict__ seg, uint8_t* __restrict__ border, int H,
-                         int W) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= H * W) return;
-  int x = idx % W, y = idx / W;
-  uint8_t b = 0;
-  if (y < 1 || x < 1 || y >= H - 1 || x >= W - 1) {
-    b = 1;
-  } else {
-    int C = seg[idx];
-    b = (C != seg[idx - W]) || (C != seg[idx + W]) || (C != seg[idx + 1]) || (C != seg[idx - 1]);
-  }
-  border[idx] = b;
-}
-
-void launch_border(const int* seg, uint8_t* border, int H, int W, hipStream_t s) {
-  k_border<<<div_up((long long)H * W, kBlock), kBlock, 0, s>>>(seg, border, H, W);
-}
-
-// ---------------------------------------------------------------------------
-// boundary update of one checkerboard phase (reference update_seg.cu:29-259)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void joint(const float* imgC, int x, int y,
-                                      const SweepRec* __restrict__ rec, int k,
-                                      float sigma2_app, float neigh_neq, float beta,
-                                      float& best, int& best_label) {
-  const SweepRec r = rec[k];
-  float res = -1000.0f;
-  const float x0 = imgC[0] - r.mu[0];
-  const float x1 = imgC[1] - r.mu[1];
-  const float x2 = imgC[2] - r.mu[2];
-  const int d0 = (int)((double)x - r.mx);
-  const int d1 = (int)((double)y - r.my);
-  res = res - (x0 * x0 + x1 * x1 + x2 * x2) / sigma2_app;
-  res = res - (float)(d0 * d0) * r.isig[0] - (float)(d1 * d1) * r.isig[2] -
-        (float)(2 * d0 * d1) * r.isig[1];
-  res = res - r.logdet;
-  res = res - beta * neigh_neq;
-  res = res - (float)(100 * (r.count <= 10));
-  if (res > best) {
-    best = res;
-    best_label = k;
-  }
-}
-
-// simple-point test + count of differing neighbours (seg_utils.h:30-79)
-__device__ __forceinline__ int nbr_check(const int nb[8], int label, uint32_t lut_bits[8],
-                                         bool& valid) {
-  int num = 0, diff = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    int same = (label == nb[i]) || (nb[i] == -1);
-    num |= same << i;
-    diff += 1 - same;
-  }
-  valid = (lut_bits[num >> 5] >> (num & 31)) & 1u;
-  return diff;
-}
-
-__global__ void k_sweep_phase(const float* __restrict__ img, int* __restrict__ seg,
-                              const uint8_t* __restrict__ border,
-                              const SweepRec* __restrict__ rec, float sigma2_app, float potts,
-                              int H, int W, int xmod, int ymod, SimpleLut lut) {
-  int hw = (W - xmod + 1) / 2;
-  int hh = (H - ymod + 1) / 2;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hw * hh) return;
-  int x = 2 * (i % hw) + xmod;
-  int y = 2 * (i / hw) + ymod;
-  int idx = y * W + x;
-  if (!border[idx]) return;
-  int C = seg[idx];
-  int N = -1, S = -1, E = -1, Wl = -1, NE = -1, NW = -1, SE = -1, SW = -1;
-  if (x > 0) Wl = seg[idx - 1];
-  if (y > 0) N = seg[idx - W];
-  if (x < W - 1) E = seg[idx + 1];
-  if (y < H - 1) S = seg[idx + W];
-  if (y > 0 && x < W - 1) NE = seg[idx - W + 1];
-  if (y > 0 && x > 0) NW = seg[idx - W - 1];
-  if (x < W - 1 && y < H - 1) SE = seg[idx + W + 1];
-  if (x > 0 && y < H - 1) SW = seg[idx + W - 1];
-  const int nb[8] = {NW, N, NE, Wl, E, SW, S, SE};
-  bool vN, vW, vS, vE;
-  int dN = nbr_check(nb, N, lut.bits, vN);
-  int dW = nbr_check(nb, Wl, lut.bits, vW);
-  int dS = nbr_check(nb, S, lut.bits, vS);
-  int dE = nbr_check(nb, E, lut.bits, vE);
-  if (!vN || !vW || !vS || !vE) return;
-  const float imgC[3] = {img[3 * idx], img[3 * idx + 1], img[3 * idx + 2]};
-  float best = -9999.0f;
-  int lab = C;
-  if (N >= 0) joint(imgC, x, y, rec, N, sigma2_app, (float)dN, potts, best, lab);
-  if (S != N && S >= 0) joint(imgC, x, y, rec, S, sigma2_app, (float)dS, potts, best, lab);
-  if (Wl != S && Wl != N && Wl >= 0)
-    joint(imgC, x, y, rec, Wl, sigma2_app, (float)dW, potts, best, lab);
-  if (E != Wl && E != S && E != N && E >= 0)
-    joint(imgC, x, y, rec, E, sigma2_app, (float)dE, potts, best, lab);
-  if (rec[C].count <= 2 && lab == C) {
-    if (N >= 0) lab = N;
-    else if (E >= 0) lab = E;
-    else if (S >= 0) lab = S;
-    else if (Wl >= 0) lab = Wl;
-  }
-  seg[idx] = lab;
-}
-
-void launch_update_seg(const float* img, int* seg, uint8_t* border, const SweepRec* rec,
-                       int niters_seg, float sigma2_app, float potts, int H, int W,
-                       const SimpleLut& lut, hipStream_t s) {
-  for (int it = 0; it < niters_seg; it++) {
-    launch_border(seg, border, H, W, s);
-    for (int xm = 0; xm < 2; xm++)
-      for (int ym = 0; ym < 2; ym++) {
-        int hw = (W - xm + 1) / 2, hh = (H - ym + 1) / 2;
-        k_sweep_phase<<<div_up((long long)hw * hh, kBlock), kBlock, 0, s>>>(
-            img, seg, border, rec, sigma2_app, potts, H, W, xm, ym, lut);
-      }
-  }
-  launch_border(seg, border, H, W, s);
-}
-
-// ---------------------------------------------------------------------------
-// sufficient statistics (reference update_params.cu:54-130), rule R1:
-// int64 fixed-point colour sums, exact position moments.
-// ---------------------------------------------------------------------------
-__global__ void k_stats_clear(LabelStats st, int n) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  st.count[k] = 0;
-  st.ninvalid[k] = 0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    st.S[3 * k + c] = 0;
-    st.Q[3 * k + c] = 0;
-  }
-#pragma unroll
-  for (int c = 0; c < 5; c++) st.X[5 * k + c] = 0;
-  if (st.bbox) {
-    st.bbox[4 * k] = 0x7fffffff;
-    st.bbox[4 * k + 1] = -1;
-    st.bbox[4 * k + 2] = 0x7fffffff;
-    st.bbox[4 * k + 3] = -1;
-  }
-}
-
-// One wave aggregates runs of equal labels along its 64 consecutive pixels
-// before touching global memory: each lane adds its pixel into the run head
-// through a segmented shuffle reduction, and only run heads issue atomics.
-template <bool BBOX>
-__global__ void k_stats_accum(const float* __restrict__ img, const int* __restrict__ seg,
-                              const int* __restrict__ shifted, int npix, int W, LabelStats st,
-                              int nlab) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  int k = -1;
-  int64_t v[11];
-#pragma unroll
-  for (int j = 0; j < 11; j++) v[j] = 0;
-  int cnt = 0, ninv = 0;
-  int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
-  if (t < npix) {
-    k = seg[t];
-    if (k >= nlab) k = -1;
-    if (k >= 0) {
-      float r = img[3 * t], g = img[3 * t + 1], b = img[3 * t + 2];
-      int64_t x = t % W, y = t / W;
-      v[0] = fx_color(r);
-      v[1] = fx_color(g);
-      v[2] = fx_color(b);
-      v[3] = fx_square(r);
-      v[4] = fx_square(g);
-      v[5] = fx_square(b);
-      v[6] = x;
-      v[7] = y;
-      v[8] = x * x;
-      v[9] = x * y;
-      v[10] = y * y;
-      cnt = 1;
-      if (shifted) ninv = shifted[t] < 0;
-      if (BBOX) {
-        bx0 = bx1 = (int)x;
-        by0 = by1 = (int)y;
-      }
-    }
-  }
-  // segmented inclusive suffix reduction towards the run head (lowest lane of
-  // the run): lane l adds lane l+d if that lane has the same label and lies
-  // in the same run.
-  int kprev = __shfl_up(k, 1);
-  bool head = (lane == 0) || (kprev != k);
-  // run id = lane of run head
-  unsigned long long heads = __ballot(head);
-  // distance to the next head above this lane
-  unsigned long long above = heads & ~((2ull << lane) - 1ull);
-  int run_end = above ? (__ffsll((long long)above) - 1) : 64;  // first lane of next run
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int c2 = __shfl_down(cnt, d);
-    int n2 = __shfl_down(ninv, d);
-    int64_t w2[11];
-#pragma unroll
-    for (int j = 0; j < 11; j++) w2[j] = __shfl_down(v[j], d);
-    int ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0;
-    if (BBOX) {
-      ax0 = __shfl_down(bx0, d);
-      ax1 = __shfl_down(bx1, d);
-      ay0 = __shfl_down(by0, d);
-      ay1 = __shfl_down(by1, d);
-    }
-    if (lane + d < run_end) {
-      cnt += c2;
-      ninv += n2;
-#pragma unroll
-      for (int j = 0; j < 11; j++) v[j] += w2[j];
-      if (BBOX) {
-        bx0 = min(bx0, ax0);
-        bx1 = max(bx1, ax1);
-        by0 = min(by0, ay0);
-        by1 = max(by1, ay1);
-      }
-    }
-  }
-  if (head && k >= 0) {
-    atomicAdd(st.count + k, cnt);
-    if (ninv) atomicAdd(st.ninvalid + k, ninv);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      atomic_add_i64(st.S + 3 * k + c, v[c]);
-      atomic_add_i64(st.Q + 3 * k + c, v[3 + c]);
-    }
-#pragma unroll
-    for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * k + c, v[6 + c]);
-    if (BBOX) {
-      atomicMin(st.bbox + 4 * k, bx0);
-      atomicMax(st.bbox + 4 * k + 1, bx1);
-      atomicMin(st.bbox + 4 * k + 2, by0);
-      atomicMax(st.bbox + 4 * k + 3, by1);
-    }
-  }
-}
-
-void launch_stats(const float* img, const int* seg, const int* shifted, int npix, int W,
-                  LabelStats st, int nlab, hipStream_t s, bool bbox) {
-  if (!bbox) st.bbox = nullptr;
-  k_stats_clear<<<div_up(nlab, kBlock), kBlock, 0, s>>>(st, nlab);
-  if (bbox)
-    k_stats_accum<true><<<div_up(npix, kBlock), kBlock, 0, s>>>(img, seg, shifted, npix, W, st, nlab);
-  else
-    k_stats_accum<false><<<div_up(npix, kBlock), kBlock, 0, s>>>(img, seg, shifted, npix, W, st,
-                                                                 nlab);
-}
-
-// ---------------------------------------------------------------------------
-// parameter updates (reference update_params.cu:141-438)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void write_rec(SweepRec* rec, const Params& P, int k) {
-  SweepRec r;
-  r.mu[0] = P.mu_app[3 * k];
-  r.mu[1] = P.mu_app[3 * k + 1];
-  r.mu[2] = P.mu_app[3 * k + 2];
-  r.logdet = (float)P.logdet[k];
-  r.isig[0] = (float)P.sigma_shape[3 * k];
-  r.isig[1] = (float)P.sigma_shape[3 * k + 1];
-  r.isig[2] = (float)P.sigma_shape[3 * k + 2];
-  r.count = P.count[k];
-  r.mx = P.mu_shape[2 * k];
-  r.my = P.mu_shape[2 * k + 1];
-  rec[k] = r;
-}
-
-// clear_fields + count copy + calc_simple_update (update_params.cu:54-88, 220-381)
-__global__ void k_param_update(Params P, LabelStats st, SweepRec* rec, int nbuf, int sp_size,
-                               int summary_mode) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nbuf) return;
-  // clear_fields
-  P.mu_app[3 * k] = P.mu_app[3 * k + 1] = P.mu_app[3 * k + 2] = 0.f;
-  P.mu_shape[2 * k] = P.mu_shape[2 * k + 1] = 0.0;
-  P.sigma_shape[3 * k] = P.sigma_shape[3 * k + 1] = P.sigma_shape[3 * k + 2] = 0.0;
-  int count_int = st.count[k];
-  P.count[k] = count_int;
-  if (P.valid[k] != 0) {
-    if (count_int <= 0) {
-      P.valid[k] = 0;
-    } else if (summary_mode) {
-      // calc_summ_stats (update_params.cu:141-203)
-      double count = count_int * 1.0;
-      for (int c = 0; c < 3; c++) P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
-      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
-      P.mu_shape[2 * k] = mx;
-      P.mu_shape[2 * k + 1] = my;
-      double sx = (double)st.X[5 * k + 2] / count - mx * mx;
-      double sy = (double)st.X[5 * k + 3] / count - mx * my;
-      double sz = (double)st.X[5 * k + 4] / count - my * my;
-      double det = sx * sz - sy * sy;
-      if (det <= 0) {
-        sx = sx + 0.00001;
-        sz = sz + 0.00001;
-        det = sx * sz - sy * sy;
-        if (det <= 0) det = 0.00001;
-      }
-      P.sigma_shape[3 * k] = sx;
-      P.sigma_shape[3 * k + 1] = sy;
-      P.sigma_shape[3 * k + 2] = sz;
-      P.logdet[k] = stsp_det_log(det);
-    } else {
-      float pc = P.prior_count[k];
-      double count = count_int * 1.0;
-      double total_count = (double)count_int + (double)(pc * 50.0f);
-      double pi[3];
-      if (P.prop[k]) {
-        // _add_sigma_smoothing(sample_sigma_shape, (int)pc, pc, sp_size)
-        int cnt = (int)pc;
-        double tc = 1. * (float)((float)cnt + pc * 50.0f);
-        const double* in = P.sample_sigma_shape + 3 * k;
-        pi[0] = ((double)(pc * (float)sp_size) + cnt * in[0]) / (tc + 3.0);
-        pi[1] = (cnt * in[1]) / (tc + 3.0);
-        pi[2] = ((double)(pc * (float)sp_size) + cnt * in[2]) / (tc + 3.0);
-      } else {
-        pi[0] = P.prior_icov[3 * k];
-        pi[1] = P.prior_icov[3 * k + 1];
-        pi[2] = P.prior_icov[3 * k + 2];
-      }
-      double pr_det = pi[0] * pi[2] - pi[1] * pi[1];
-      pr_det = sqrt(pr_det);
-      double ps0 = (double)pc / pr_det * pi[0];
-      double ps1 = (double)pc / pr_det * pi[1];
-      double ps2 = (double)pc / pr_det * pi[2];
-      for (int c = 0; c < 3; c++) P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
-      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
-      P.mu_shape[2 * k] = mx;
-      P.mu_shape[2 * k + 1] = my;
-      double sx = (double)st.X[5 * k + 2] - count * mx * mx;
-      double sy = (double)st.X[5 * k + 3] - count * mx * my;
-      double sz = (double)st.X[5 * k + 4] - count * my * my;
-      sx = ((double)pc * ps0 + sx) / (total_count + 3.0);
-      sy = ((double)pc * ps1 + sy) / (total_count + 3.0);
-      sz = ((double)pc * ps2 + sz) / (total_count + 3.0);
-      double det = sx * sz - sy * sy;
-      if (det <= 0) {
-        sx = sx + 0.00001;
-        sz = sz + 0.00001;
-        det = sx * sz - sy * sy;
-        if (det <= 0) det = 0.00001;
-      }
-      P.sigma_shape[3 * k] = sz / det;
-      P.sigma_shape[3 * k + 1] = -sy / det;
-      P.sigma_shape[3 * k + 2] = sx / det;
-      P.logdet[k] = stsp_det_log(det);
-    }
-  }
-  if (rec) write_rec(rec, P, k);
-}
-
-void launch_param_update(Params P, LabelStats st, SweepRec* rec, int nbuf, int sp_size,
-                         int summary_mode, hipStream_t s) {
-  k_param_update<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, st, rec, nbuf, sp_size,
-                                                         summary_mode);
-}
-
-// store_sample_sigma_shape_k (update_params.cu:396-438)
-__global__ void k_store_sample(Params P, LabelStats st, int nbuf) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nbuf || P.valid[k] == 0) return;
-  double count = P.count[k] * 1.0;
-  double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
-  P.mu_shape[2 * k] = mx;
-  P.mu_shape[2 * k + 1] = my;
-  if (count > 0) {
-    P.sample_sigma_shape[3 * k] = (double)st.X[5 * k + 2] / count - mx * mx;
-    P.sample_sigma_shape[3 * k + 1] = (double)st.X[5 * k + 3] / count - mx * my;
-    P.sample_sigma_shape[3 * k + 2] = (double)st.X[5 * k + 4] / count - my * my;
-  } else {
-    for (int c = 0; c < 3; c++) P.sample_sigma_shape[3 * k + c] = P.prior_sigma_shape[3 * k + c];
-  }
-}
-
-void launch_store_sample(Params P, LabelStats st, int nbuf, hipStream_t s) {
-  k_store_sample<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, st, nbuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -506,12 +121,11 @@ __global__ void k_bass_init(Params P, int nspix, int nbuf, int sp_size, int npix
   }
 }
 
-void launch_bass_params_init(Params P, int nspix, int nbuf, int sp_size, int npix,
-                             LabelStats st, const float* img, const int* seg, int W,
-                             hipStream_t s) {
+void launch_bass_mark(Params P, int nspix, int nbuf, int sp_size, hipStream_t s) {
   k_bass_mark<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, nspix, nbuf, sp_size);
-  launch_stats(img, seg, nullptr, npix, W, st, nbuf, s);
-  launch_param_update(P, st, nullptr, nbuf, sp_size, 1, s);
+}
+
+void launch_bass_init(Params P, int nspix, int nbuf, int sp_size, int npix, hipStream_t s) {
   k_bass_init<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, nspix, nbuf, sp_size, npix);
 }
 

@@ -1,0 +1,1102 @@
+// Batched refinement-loop kernels: every launch covers all frames of a batch
+// (see batch.h).  Semantics per frame are those of the reference kernels cited
+// at each function.  Where the reference depends on atomic arrival order the
+// rule here is order-independent:
+//   - colour/position sums are exact int64 fixed-point additions;
+//   - the merge winner of a target f is the largest accepting k (atomicMax);
+//   - new split / relabel ids are max_spix+1+rank in ascending label order;
+//   - remove steps read valid / prior_count / pairs from a snapshot;
+//   - the relabel candidate is the lexicographic minimum of (distance, id).
+#include "batch.h"
+
+namespace stsp {
+
+__device__ __forceinline__ int frame_of(const int* __restrict__ off, int nf, int b) {
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// NIG marginal (split_merge_prop.cu:2975-3019); lgamma terms cancel exactly.
+__device__ __forceinline__ double nig_marg(const NigConsts& c, double n, const double* bn) {
+  double a_n = c.a0 + n / 2.0;
+  double num = a_n * c.logb0 + 0.5 * stsp_det_log(1.0 / n);
+  double t = 0.5 * n * c.logpi + n * c.log2;
+  double m = 0.0;
+  for (int ch = 0; ch < 3; ch++) {
+    double den = a_n * stsp_det_log(bn[ch]) + t;
+    m = m + (num - den);
+  }
+  return m;
+}
+
+#define PIX_PROLOGUE                                         \
+  const int f = frame_of(b.off.pix, b.nf, blockIdx.x);       \
+  const FrameDesc& F = b.fr[f];                              \
+  const int t = (blockIdx.x - b.off.pix[f]) * kBlock + threadIdx.x;
+
+#define LAB_PROLOGUE                                         \
+  const int f = frame_of(b.off.lab, b.nf, blockIdx.x);       \
+  const FrameDesc& F = b.fr[f];                              \
+  const int k = (blockIdx.x - b.off.lab[f]) * kBlock + threadIdx.x;
+
+#define SM_PROLOGUE                                          \
+  const int f = frame_of(b.off.sm, b.nf, blockIdx.x);        \
+  const FrameDesc& F = b.fr[f];                              \
+  const int k = (blockIdx.x - b.off.sm[f]) * kBlock + threadIdx.x;
+
+// ---------------------------------------------------------------------------
+// statistics (update_params.cu:54-130), rule R1
+// ---------------------------------------------------------------------------
+__global__ void kb_stats_clear(Batch b, int bbox) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  LabelStats st = F.st;
+  st.count[k] = 0;
+  st.ninvalid[k] = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    st.S[3 * k + c] = 0;
+    st.Q[3 * k + c] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) st.X[5 * k + c] = 0;
+  if (bbox) {
+    st.bbox[4 * k] = 0x7fffffff;
+    st.bbox[4 * k + 1] = -1;
+    st.bbox[4 * k + 2] = 0x7fffffff;
+    st.bbox[4 * k + 3] = -1;
+  }
+}
+
+// which: 0 = live labels into st (limit nbuf); 1 = split halves into sst
+// (limit nsm, ninvalid from the shifted map); 2 = live labels into sst.
+template <bool BBOX>
+__global__ void kb_stats_accum(Batch b, int cur, int which) {
+  PIX_PROLOGUE
+  const int lane = threadIdx.x & 63;
+  const int* labs = which == 1 ? F.halves : F.seg[cur];
+  LabelStats st = which == 0 ? F.st : F.sst;
+  const int nlab = which == 0 ? F.nbuf : F.nsm;
+  const int* shifted = (which == 1 && F.prop_mode) ? F.shifted : nullptr;
+  const int W = F.W;
+  int k = -1;
+  int64_t v[11];
+#pragma unroll
+  for (int j = 0; j < 11; j++) v[j] = 0;
+  int cnt = 0, ninv = 0;
+  int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
+  if (t < F.npix) {
+    k = labs[t];
+    if (k >= nlab) k = -1;
+    if (k >= 0) {
+      const float* p = F.img + 3 * t;
+      float r = p[0], g = p[1], bl = p[2];
+      int64_t x = t % W, y = t / W;
+      v[0] = fx_color(r);
+      v[1] = fx_color(g);
+      v[2] = fx_color(bl);
+      v[3] = fx_square(r);
+      v[4] = fx_square(g);
+      v[5] = fx_square(bl);
+      v[6] = x;
+      v[7] = y;
+      v[8] = x * x;
+      v[9] = x * y;
+      v[10] = y * y;
+      cnt = 1;
+      if (shifted) ninv = shifted[t] < 0;
+      if (BBOX) {
+        bx0 = bx1 = (int)x;
+        by0 = by1 = (int)y;
+      }
+    }
+  }
+  int run_end;
+  const bool head = wave_run(k, run_end);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int c2 = __shfl_down(cnt, d);
+    int n2 = __shfl_down(ninv, d);
+    int64_t w2[11];
+#pragma unroll
+    for (int j = 0; j < 11; j++) w2[j] = __shfl_down(v[j], d);
+    int ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0;
+    if (BBOX) {
+      ax0 = __shfl_down(bx0, d);
+      ax1 = __shfl_down(bx1, d);
+      ay0 = __shfl_down(by0, d);
+      ay1 = __shfl_down(by1, d);
+    }
+    if (lane + d < run_end) {
+      cnt += c2;
+      ninv += n2;
+#pragma unroll
+      for (int j = 0; j < 11; j++) v[j] += w2[j];
+      if (BBOX) {
+        bx0 = min(bx0, ax0);
+        bx1 = max(bx1, ax1);
+        by0 = min(by0, ay0);
+        by1 = max(by1, ay1);
+      }
+    }
+  }
+  if (head && k >= 0) {
+    atomicAdd(st.count + k, cnt);
+    if (ninv) atomicAdd(st.ninvalid + k, ninv);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      atomic_add_i64(st.S + 3 * k + c, v[c]);
+      atomic_add_i64(st.Q + 3 * k + c, v[3 + c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * k + c, v[6 + c]);
+    if (BBOX) {
+      atomicMin(st.bbox + 4 * k, bx0);
+      atomicMax(st.bbox + 4 * k + 1, bx1);
+      atomicMin(st.bbox + 4 * k + 2, by0);
+      atomicMax(st.bbox + 4 * k + 3, by1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// parameter update (update_params.cu:54-88, 141-203, 220-381)
+// ---------------------------------------------------------------------------
+__global__ void kb_param_update(Batch b, int summary_mode) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  Params P = F.P;
+  LabelStats st = F.st;
+  const int sp_size = F.sp_size;
+  P.mu_app[3 * k] = P.mu_app[3 * k + 1] = P.mu_app[3 * k + 2] = 0.f;
+  P.mu_shape[2 * k] = P.mu_shape[2 * k + 1] = 0.0;
+  P.sigma_shape[3 * k] = P.sigma_shape[3 * k + 1] = P.sigma_shape[3 * k + 2] = 0.0;
+  const int count_int = st.count[k];
+  P.count[k] = count_int;
+  if (P.valid[k] != 0) {
+    if (count_int <= 0) {
+      P.valid[k] = 0;
+    } else if (summary_mode) {
+      double count = count_int * 1.0;
+      for (int c = 0; c < 3; c++)
+        P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
+      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
+      P.mu_shape[2 * k] = mx;
+      P.mu_shape[2 * k + 1] = my;
+      double sx = (double)st.X[5 * k + 2] / count - mx * mx;
+      double sy = (double)st.X[5 * k + 3] / count - mx * my;
+      double sz = (double)st.X[5 * k + 4] / count - my * my;
+      double det = sx * sz - sy * sy;
+      if (det <= 0) {
+        sx = sx + 0.00001;
+        sz = sz + 0.00001;
+        det = sx * sz - sy * sy;
+        if (det <= 0) det = 0.00001;
+      }
+      P.sigma_shape[3 * k] = sx;
+      P.sigma_shape[3 * k + 1] = sy;
+      P.sigma_shape[3 * k + 2] = sz;
+      P.logdet[k] = stsp_det_log(det);
+    } else {
+      float pc = P.prior_count[k];
+      double count = count_int * 1.0;
+      double total_count = (double)count_int + (double)(pc * 50.0f);
+      double pi[3];
+      if (P.prop[k]) {
+        int cnt = (int)pc;  // _add_sigma_smoothing(sample_sigma_shape, (int)pc, pc, sp_size)
+        double tc = 1. * (float)((float)cnt + pc * 50.0f);
+        const double* in = P.sample_sigma_shape + 3 * k;
+        pi[0] = ((double)(pc * (float)sp_size) + cnt * in[0]) / (tc + 3.0);
+        pi[1] = (cnt * in[1]) / (tc + 3.0);
+        pi[2] = ((double)(pc * (float)sp_size) + cnt * in[2]) / (tc + 3.0);
+      } else {
+        pi[0] = P.prior_icov[3 * k];
+        pi[1] = P.prior_icov[3 * k + 1];
+        pi[2] = P.prior_icov[3 * k + 2];
+      }
+      double pr_det = pi[0] * pi[2] - pi[1] * pi[1];
+      pr_det = sqrt(pr_det);
+      double ps0 = (double)pc / pr_det * pi[0];
+      double ps1 = (double)pc / pr_det * pi[1];
+      double ps2 = (double)pc / pr_det * pi[2];
+      for (int c = 0; c < 3; c++)
+        P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
+      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
+      P.mu_shape[2 * k] = mx;
+      P.mu_shape[2 * k + 1] = my;
+      double sx = (double)st.X[5 * k + 2] - count * mx * mx;
+      double sy = (double)st.X[5 * k + 3] - count * mx * my;
+      double sz = (double)st.X[5 * k + 4] - count * my * my;
+      sx = ((double)pc * ps0 + sx) / (total_count + 3.0);
+      sy = ((double)pc * ps1 + sy) / (total_count + 3.0);
+      sz = ((double)pc * ps2 + sz) / (total_count + 3.0);
+      double det = sx * sz - sy * sy;
+      if (det <= 0) {
+        sx = sx + 0.00001;
+        sz = sz + 0.00001;
+        det = sx * sz - sy * sy;
+        if (det <= 0) det = 0.00001;
+      }
+      P.sigma_shape[3 * k] = sz / det;
+      P.sigma_shape[3 * k + 1] = -sy / det;
+      P.sigma_shape[3 * k + 2] = sx / det;
+      P.logdet[k] = stsp_det_log(det);
+    }
+  }
+  SweepRec r;
+  r.mu[0] = P.mu_app[3 * k];
+  r.mu[1] = P.mu_app[3 * k + 1];
+  r.mu[2] = P.mu_app[3 * k + 2];
+  r.logdet = (float)P.logdet[k];
+  r.isig[0] = (float)P.sigma_shape[3 * k];
+  r.isig[1] = (float)P.sigma_shape[3 * k + 1];
+  r.isig[2] = (float)P.sigma_shape[3 * k + 2];
+  r.count = P.count[k];
+  r.mx = P.mu_shape[2 * k];
+  r.my = P.mu_shape[2 * k + 1];
+  F.rec[k] = r;
+}
+
+// store_sample_sigma_shape_k (update_params.cu:396-438)
+__global__ void kb_store_sample(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  Params P = F.P;
+  LabelStats st = F.st;
+  if (P.valid[k] == 0) return;
+  double count = P.count[k] * 1.0;
+  double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
+  P.mu_shape[2 * k] = mx;
+  P.mu_shape[2 * k + 1] = my;
+  if (count > 0) {
+    P.sample_sigma_shape[3 * k] = (double)st.X[5 * k + 2] / count - mx * mx;
+    P.sample_sigma_shape[3 * k + 1] = (double)st.X[5 * k + 3] / count - mx * my;
+    P.sample_sigma_shape[3 * k + 2] = (double)st.X[5 * k + 4] / count - my * my;
+  } else {
+    for (int c = 0; c < 3; c++) P.sample_sigma_shape[3 * k + c] = P.prior_sigma_shape[3 * k + c];
+  }
+}
+
+void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
+  kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, bbox ? 1 : 0);
+  if (bbox)
+    kb_stats_accum<true><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+  else
+    kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+  kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
+}
+
+void b_param_summary(const Batch& b, int cur, hipStream_t s) {
+  kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
+  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+  kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 1);
+}
+
+void b_store_sample(const Batch& b, hipStream_t s) {
+  kb_store_sample<<<b.lab_blocks, kBlock, 0, s>>>(b);
+}
+
+// ---------------------------------------------------------------------------
+// fused boundary sweeps (update_seg.cu:29-311): S sweeps per launch in LDS
+// tiles with a 4S halo (see k_sweep_fused's description in k_sweep.hip)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void joint_b(const float* imgC, int x, int y, const SweepRec& r, int k,
+                                        float sigma2_app, float neigh_neq, float beta,
+                                        float& best, int& best_label) {
+  float res = -1000.0f;
+  const float x0 = imgC[0] - r.mu[0];
+  const float x1 = imgC[1] - r.mu[1];
+  const float x2 = imgC[2] - r.mu[2];
+  const int d0 = (int)((double)x - r.mx);
+  const int d1 = (int)((double)y - r.my);
+  res = res - (x0 * x0 + x1 * x1 + x2 * x2) / sigma2_app;
+  res = res - (float)(d0 * d0) * r.isig[0] - (float)(d1 * d1) * r.isig[2] -
+        (float)(2 * d0 * d1) * r.isig[1];
+  res = res - r.logdet;
+  res = res - beta * neigh_neq;
+  res = res - (float)(100 * (r.count <= 10));
+  if (res > best) {
+    best = res;
+    best_label = k;
+  }
+}
+
+__device__ __forceinline__ int nbits(const int nb[8], int label) {
+  int num = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) num |= ((label == nb[i]) || (nb[i] == -1)) << i;
+  return num;
+}
+
+template <int TW, int TH, int S, int NT, int SHAPE>
+__global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) {
+  constexpr int R = 4 * S;
+  constexpr int LW = TW + 2 * R, LH = TH + 2 * R;
+  __shared__ int lab[LH * LW];
+  __shared__ uint8_t bord[LH * LW];
+  __shared__ uint32_t lutb[8];
+  const int f = frame_of(b.off.tile[SHAPE], b.nf, blockIdx.x);
+  const FrameDesc& F = b.fr[f];
+  const int H = F.H, W = F.W;
+  const float* __restrict__ img = F.img;
+  const int* __restrict__ seg_in = F.seg[cur];
+  int* __restrict__ seg_out = F.seg[cur ^ 1];
+  const SweepRec* __restrict__ rec = F.rec;
+  const float sigma2_app = F.sigma2_app, potts = F.potts;
+  const int tiles_x = (W + TW - 1) / TW;
+  const int tile = blockIdx.x - b.off.tile[SHAPE][f];
+  const int tid = threadIdx.x;
+  if (tid < 8) lutb[tid] = lut.bits[tid];
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int x0 = tx * TW - R, y0 = ty * TH - R;
+  for (int i = tid; i < LH * LW; i += NT) {
+    int lx = i % LW, ly = i / LW;
+    int gx = x0 + lx, gy = y0 + ly;
+    lab[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? seg_in[gy * W + gx] : -1;
+  }
+  __syncthreads();
+  for (int s = 0; s < S; s++) {
+    {
+      const int rb = 4 * (S - 1 - s) + 3;
+      const int bx0 = R - rb, by0 = R - rb, bw = TW + 2 * rb, bh = TH + 2 * rb;
+      for (int i = tid; i < bw * bh; i += NT) {
+        int lx = bx0 + i % bw, ly = by0 + i / bw;
+        int gx = x0 + lx, gy = y0 + ly;
+        int li = ly * LW + lx;
+        uint8_t bb = 0;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+          if (gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1) {
+            bb = 1;
+          } else {
+            int C = lab[li];
+            bb = (C != lab[li - LW]) || (C != lab[li + LW]) || (C != lab[li + 1]) ||
+                 (C != lab[li - 1]);
+          }
+        }
+        bord[li] = bb;
+      }
+      __syncthreads();
+    }
+#pragma unroll 1
+    for (int p = 0; p < 4; p++) {
+      const int xm = p >> 1, ym = p & 1;
+      const int rp = 4 * (S - 1 - s) + (3 - p);
+      const int rx0 = R - rp, ry0 = R - rp, rw = TW + 2 * rp, rh = TH + 2 * rp;
+      const int px0 = rx0 + (((xm - (x0 + rx0)) % 2) + 2) % 2;
+      const int py0 = ry0 + (((ym - (y0 + ry0)) % 2) + 2) % 2;
+      const int nx = (rx0 + rw - px0 + 1) / 2, ny = (ry0 + rh - py0 + 1) / 2;
+      for (int i = tid; i < nx * ny; i += NT) {
+        const int lx = px0 + 2 * (i % nx), ly = py0 + 2 * (i / nx);
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+        const int li = ly * LW + lx;
+        if (!bord[li]) continue;
+        const int C = lab[li];
+        const int N = lab[li - LW], Sd = lab[li + LW], Wl = lab[li - 1], E = lab[li + 1];
+        const int nb[8] = {lab[li - LW - 1], N, lab[li - LW + 1], Wl, E, lab[li + LW - 1], Sd,
+                           lab[li + LW + 1]};
+        const int bN = nbits(nb, N), bW = nbits(nb, Wl), bS = nbits(nb, Sd), bE = nbits(nb, E);
+        const bool ok = ((lutb[bN >> 5] >> (bN & 31)) & 1u) && ((lutb[bW >> 5] >> (bW & 31)) & 1u) &&
+                        ((lutb[bS >> 5] >> (bS & 31)) & 1u) && ((lutb[bE >> 5] >> (bE & 31)) & 1u);
+        if (!ok) continue;
+        const int gidx = gy * W + gx;
+        const float imgC[3] = {img[3 * gidx], img[3 * gidx + 1], img[3 * gidx + 2]};
+        float best = -9999.0f;
+        int out = C;
+        if (N >= 0) joint_b(imgC, gx, gy, rec[N], N, sigma2_app, (float)(8 - __popc(bN)), potts, best, out);
+        if (Sd != N && Sd >= 0)
+          joint_b(imgC, gx, gy, rec[Sd], Sd, sigma2_app, (float)(8 - __popc(bS)), potts, best, out);
+        if (Wl != Sd && Wl != N && Wl >= 0)
+          joint_b(imgC, gx, gy, rec[Wl], Wl, sigma2_app, (float)(8 - __popc(bW)), potts, best, out);
+        if (E != Wl && E != Sd && E != N && E >= 0)
+          joint_b(imgC, gx, gy, rec[E], E, sigma2_app, (float)(8 - __popc(bE)), potts, best, out);
+        if (rec[C].count <= 2 && out == C) {
+          if (N >= 0) out = N;
+          else if (E >= 0) out = E;
+          else if (Sd >= 0) out = Sd;
+          else if (Wl >= 0) out = Wl;
+        }
+        lab[li] = out;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < TW * TH; i += NT) {
+    int lx = R + i % TW, ly = R + i / TW;
+    int gx = x0 + lx, gy = y0 + ly;
+    if (gx < W && gy < H) seg_out[gy * W + gx] = lab[ly * LW + lx];
+  }
+}
+
+// runs niters_seg sweeps; returns the index of the buffer holding the result
+int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut& lut,
+             hipStream_t s) {
+  int done = 0;
+  while (done < niters_seg) {
+    int left = niters_seg - done;
+    if (shape == 2 && left >= 4) {
+      kb_sweep<64, 32, 4, 256, 2><<<b.tile_blocks[2], 256, 0, s>>>(b, cur, lut);
+      done += 4;
+    } else if (shape >= 1 && left >= 2) {
+      kb_sweep<32, 32, 2, 256, 1><<<b.tile_blocks[1], 256, 0, s>>>(b, cur, lut);
+      done += 2;
+    } else {
+      kb_sweep<32, 16, 1, 64, 0><<<b.tile_blocks[0], 64, 0, s>>>(b, cur, lut);
+      done += 1;
+    }
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// ---------------------------------------------------------------------------
+// per-frame helpers
+// ---------------------------------------------------------------------------
+// one workgroup per frame: exclusive scan of n = nbuf entries; which selects
+// (split accept -> sm.rank) or (relabel want_new -> rank); total -> F.total
+__global__ void __launch_bounds__(1024) kb_scan(Batch b, int which) {
+  const FrameDesc& F = b.fr[blockIdx.x];
+  const int* in = which == 0 ? F.sm.accept : F.want_new;
+  int* out = which == 0 ? F.sm.rank : F.rank;
+  const int n = F.nbuf;
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int beg = tid * chunk, end = min(n, beg + chunk);
+  int sum = 0;
+  for (int i = beg; i < end; i++) sum += in[i];
+  part[tid] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    int v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int i = beg; i < end; i++) {
+    int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (tid == 1023) *F.total = part[1023];
+}
+
+__global__ void kb_reset_work(Batch b) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    b.work_n[0] = 0;
+    b.work_n[1] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// split (split_merge_orig.cu:202-420, split_merge_prop.cu:184-395)
+// ---------------------------------------------------------------------------
+__global__ void kb_sm_clear(Batch b) {
+  SM_PROLOGUE
+  if (k >= F.nsm) return;
+  LabelStats sst = F.sst;
+  SmState sm = F.sm;
+  sst.count[k] = 0;
+  sst.ninvalid[k] = 0;
+  for (int c = 0; c < 3; c++) {
+    sst.S[3 * k + c] = 0;
+    sst.Q[3 * k + c] = 0;
+    sm.b_n[3 * k + c] = 0.0;
+    sm.b_n_f[3 * k + c] = 0.0;
+  }
+  for (int c = 0; c < 5; c++) sst.X[5 * k + c] = 0;
+  sm.count_f[k] = 0;
+  sm.has_bn[k] = 0;
+  sm.marg[k] = 0.0;
+  sm.hasting[k] = -999999.0;
+  sm.merge[k] = 0;
+  sm.remove[k] = 0;
+  sm.select[k] = 0;
+  sm.accept[k] = 0;
+  sm.winner[k] = -1;
+  if (k < F.nbuf) {
+    sm.pairs[2 * k] = -1;
+    sm.pairs[2 * k + 1] = -1;
+  }
+}
+
+__global__ void kb_copy_halves(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t < F.npix) F.halves[t] = F.seg[cur][t];
+}
+
+// labels that take part in the split: valid, non-empty, split not skipped
+__global__ void kb_split_list(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  if (*F.max_spix > F.nbuf / 2 || F.P.valid[k] == 0 || F.P.count[k] <= 0) return;
+  int pos = atomicAdd(b.work_n, 1);
+  if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
+}
+
+__device__ __forceinline__ int seed_of(const FrameDesc& F, const int* seg, int k, int dir) {
+  int x, y;
+  if (dir == 1 || dir == -1) {
+    x = (int)(F.P.mu_shape[2 * k]) + dir;
+    y = (int)(F.P.mu_shape[2 * k + 1]);
+  } else {
+    x = (int)(F.P.mu_shape[2 * k]);
+    y = (int)(F.P.mu_shape[2 * k + 1]) + dir;
+  }
+  int ind = y * F.W + x;
+  if (ind < 0 || ind > F.W * F.H - 1) return -1;
+  return seg[ind] == k ? ind : -1;
+}
+
+constexpr int kBfsCapB = 8192;
+
+// Persistent workgroups pull labels from the work list and run the two-seed
+// level-synchronous BFS of calc_split_candidate (split_merge.cu:757-800)
+// inside the label's bounding box (LDS; global maps for oversized boxes),
+// then mark the far half (calc_seg_split 847-860).
+__global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
+  __shared__ short d1[kBfsCapB], d2[kBfsCapB];
+  __shared__ uint8_t m[kBfsCapB];
+  __shared__ int item, changed;
+  const int tid = threadIdx.x;
+  const int n_items = min(b.work_n[0], b.work_cap);
+  while (true) {
+    __syncthreads();
+    if (tid == 0) item = atomicAdd(b.work_n + 1, 1);
+    __syncthreads();
+    const int it = item;
+    if (it >= n_items) break;
+    const int wv = b.work[it];
+    const int f = wv >> 24, k = wv & 0xffffff;
+    const FrameDesc& F = b.fr[f];
+    const int* seg = F.seg[cur];
+    const int W = F.W;
+    const int* bbox = F.st.bbox;
+    const int bx0 = bbox[4 * k], bx1 = bbox[4 * k + 1], by0 = bbox[4 * k + 2], by1 = bbox[4 * k + 3];
+    if (bx1 < bx0 || by1 < by0) continue;
+    const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1, nc = bw * bh;
+    const int s1 = seed_of(F, seg, k, direction);
+    const int s2 = seed_of(F, seg, k, -direction);
+    const int k2 = k + *F.max_spix + 1;
+    int* halves = F.halves;
+    if (nc <= kBfsCapB) {
+      for (int i = tid; i < nc; i += 256) {
+        int g = (by0 + i / bw) * W + bx0 + i % bw;
+        m[i] = seg[g] == k;
+        d1[i] = (g == s1) ? 1 : -1;
+        d2[i] = (g == s2) ? 1 : -1;
+      }
+      __syncthreads();
+      for (int level = 1;; level++) {
+        if (tid == 0) changed = 0;
+        __syncthreads();
+        int ch = 0;
+        for (int i = tid; i < nc; i += 256) {
+          if (!m[i]) continue;
+          bool u1 = d1[i] == -1, u2 = d2[i] == -1;
+          if (!u1 && !u2) continue;
+          int lx = i % bw, ly = i / bw;
+          bool h1 = false, h2 = false;
+          if (ly > 0 && m[i - bw]) { h1 |= d1[i - bw] == level; h2 |= d2[i - bw] == level; }
+          if (lx > 0 && m[i - 1]) { h1 |= d1[i - 1] == level; h2 |= d2[i - 1] == level; }
+          if (ly < bh - 1 && m[i + bw]) { h1 |= d1[i + bw] == level; h2 |= d2[i + bw] == level; }
+          if (lx < bw - 1 && m[i + 1]) { h1 |= d1[i + 1] == level; h2 |= d2[i + 1] == level; }
+          if (u1 && h1) { d1[i] = (short)(level + 1); ch = 1; }
+          if (u2 && h2) { d2[i] = (short)(level + 1); ch = 1; }
+        }
+        if (ch) changed = 1;
+        __syncthreads();
+        const int go = changed;
+        __syncthreads();
+        if (!go) break;
+      }
+      for (int i = tid; i < nc; i += 256)
+        if (m[i] && d1[i] > d2[i]) halves[(by0 + i / bw) * W + bx0 + i % bw] = k2;
+      continue;
+    }
+    int* d1g = F.d1;
+    int* d2g = F.d2;
+    for (int i = tid; i < nc; i += 256) {
+      int g = (by0 + i / bw) * W + bx0 + i % bw;
+      if (seg[g] != k) continue;
+      d1g[g] = (g == s1) ? 1 : -1;
+      d2g[g] = (g == s2) ? 1 : -1;
+    }
+    __syncthreads();
+    for (int level = 1;; level++) {
+      if (tid == 0) changed = 0;
+      __syncthreads();
+      int ch = 0;
+      for (int i = tid; i < nc; i += 256) {
+        int lx = i % bw, ly = i / bw, g = (by0 + ly) * W + bx0 + lx;
+        if (seg[g] != k) continue;
+        bool u1 = d1g[g] == -1, u2 = d2g[g] == -1;
+        if (!u1 && !u2) continue;
+        bool h1 = false, h2 = false;
+        int nb[4] = {ly > 0 ? g - W : -1, lx > 0 ? g - 1 : -1, ly < bh - 1 ? g + W : -1,
+                     lx < bw - 1 ? g + 1 : -1};
+        for (int j = 0; j < 4; j++) {
+          int q = nb[j];
+          if (q < 0 || seg[q] != k) continue;
+          h1 |= d1g[q] == level;
+          h2 |= d2g[q] == level;
+        }
+        if (u1 && h1) { d1g[g] = level + 1; ch = 1; }
+        if (u2 && h2) { d2g[g] = level + 1; ch = 1; }
+      }
+      if (ch) changed = 1;
+      __syncthreads();
+      const int go = changed;
+      __syncthreads();
+      if (!go) break;
+    }
+    for (int i = tid; i < nc; i += 256) {
+      int g = (by0 + i / bw) * W + bx0 + i % bw;
+      if (seg[g] == k && d1g[g] > d2g[g]) halves[g] = k2;
+    }
+  }
+}
+
+__global__ void kb_split_eval(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const int ms = *F.max_spix;
+  const Params& P = F.P;
+  if (ms > F.nbuf / 2 || P.valid[k] == 0) return;
+  const int s = k + (ms + 1);
+  if (s >= F.nbuf) return;
+  const LabelStats& st = F.st;
+  const LabelStats& sst = F.sst;
+  const NigConsts& nc = F.nc;
+  int count_f = P.count[k], count_k = sst.count[k], count_s = sst.count[s];
+  if (count_f < 1 || count_k < 1 || count_s < 1) return;
+  if (count_f != count_k + count_s) return;
+  double bk[3], bs[3], bf[3];
+  for (int c = 0; c < 3; c++) {
+    double Qk = fx_to_double(sst.Q[3 * k + c]), Qs = fx_to_double(sst.Q[3 * s + c]);
+    double Sk = fx_to_double(sst.S[3 * k + c]), Ss = fx_to_double(sst.S[3 * s + c]);
+    double Sf = fx_to_double(st.S[3 * k + c]);
+    bk[c] = nc.b0 + 0.5 * (Qk - (Sk * Sk) / (double)count_k);
+    bs[c] = nc.b0 + 0.5 * (Qs - (Ss * Ss) / (double)count_s);
+    bf[c] = nc.b0 + 0.5 * ((Qk + Qs) - (Sf * Sf) / (double)count_f);
+  }
+  double h = nig_marg(nc, (double)count_k, bk) + nig_marg(nc, (double)count_s, bs) -
+             nig_marg(nc, (double)count_f, bf) + (double)F.split_alpha;
+  if (F.prop_mode) {
+    int is_prop = P.prop[k];
+    int nk = sst.ninvalid[k], ns = sst.ninvalid[s];
+    int propogate_k = (nk <= ns);
+    F.sm.select[k] = propogate_k || !is_prop;
+    double iperc_k = nk / (1.0 * count_k);
+    double iperc_s = nk / (1.0 * count_s);
+    double iperc = propogate_k ? iperc_k : iperc_s;
+    iperc = is_prop ? iperc : 1.0;
+    h = h - 2.0 * (1.0 - iperc);
+  }
+  F.sm.hasting[k] = h;
+  F.sm.accept[k] = h > -2.0 ? 1 : 0;
+}
+
+__global__ void kb_split_assign(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const int ms = *F.max_spix;
+  if (ms > F.nbuf / 2 || !F.sm.accept[k]) return;
+  Params P = F.P;
+  const int s = ms + 1 + F.sm.rank[k];
+  F.sm.pairs[2 * k] = s;
+  if (!F.prop_mode) {
+    float pc = (float)fmax(P.prior_count[k] / 2.0, 8.0);
+    P.prior_count[k] = pc;
+    P.prior_count[s] = pc;
+  } else {
+    int nk = F.sst.ninvalid[k], ns = F.sst.ninvalid[k + ms + 1];
+    int prop_k = (nk <= ns);
+    float prior_count = P.prior_count[k];
+    float half = (float)fmax(prior_count / 2.0, 36.0);
+    float pcs;
+    if (prop_k) {
+      P.prior_count[k] = prior_count;
+      pcs = (float)(prior_count / 2.);
+    } else {
+      P.prior_count[k] = half;
+      pcs = half;
+      P.prior_icov[3 * k] = pcs;
+      P.prior_icov[3 * k + 1] = 0;
+      P.prior_icov[3 * k + 2] = pcs;
+    }
+    P.prior_count[s] = pcs;
+    P.prop[s] = 0;
+    P.icount[k] = (float)(P.icount[k] / 2.0);
+    P.prior_icov[3 * s] = pcs;
+    P.prior_icov[3 * s + 1] = 0;
+    P.prior_icov[3 * s + 2] = pcs;
+    P.valid[s] = 1;
+  }
+}
+
+__global__ void kb_split_apply(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t >= F.npix) return;
+  const int ms = *F.max_spix;
+  if (ms > F.nbuf / 2) return;
+  int* seg = F.seg[cur];
+  int k = seg[t];
+  if (k < 0 || k >= F.nbuf || !F.sm.accept[k]) return;
+  const int k2 = k + (ms + 1);
+  const int h = F.halves[t];
+  const bool take = F.prop_mode ? (F.sm.select[k] ? (h == k2) : (h == k)) : (h == k2);
+  if (take) {
+    int s = F.sm.pairs[2 * k];
+    seg[t] = s;
+    F.P.valid[s] = 1;
+  }
+}
+
+__global__ void kb_add_max(Batch b) {
+  int f = threadIdx.x;
+  if (f >= b.nf) return;
+  const FrameDesc& F = b.fr[f];
+  if (*F.max_spix > F.nbuf / 2) return;
+  *F.max_spix += *F.total;
+}
+
+void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
+  kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
+  kb_copy_halves<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_reset_work<<<1, 64, 0, s>>>(b);
+  kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_bfs<<<1024, 256, 0, s>>>(b, cur, direction);
+  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 1);
+  kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
+  kb_split_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_split_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_add_max<<<1, 64, 0, s>>>(b);
+}
+
+// ---------------------------------------------------------------------------
+// merge (split_merge_orig.cu:79-199, split_merge_prop.cu:87-181)
+// ---------------------------------------------------------------------------
+__global__ void kb_border(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t >= F.npix) return;
+  const int W = F.W, H = F.H;
+  const int* seg = F.seg[cur];
+  int x = t % W, y = t / W;
+  uint8_t bb;
+  if (y < 1 || x < 1 || y >= H - 1 || x >= W - 1) {
+    bb = 1;
+  } else {
+    int C = seg[t];
+    bb = (C != seg[t - W]) || (C != seg[t + W]) || (C != seg[t + 1]) || (C != seg[t - 1]);
+  }
+  F.border[t] = bb;
+}
+
+__global__ void kb_merge_cand(Batch b, int cur, int direction) {
+  PIX_PROLOGUE
+  if (t >= F.npix || !F.border[t]) return;
+  const int W = F.W, H = F.H;
+  const int* seg = F.seg[cur];
+  const int prop_mode = F.prop_mode;
+  int x = t % W, y = t / W;
+  int C = seg[t], Wn = -1;
+  if (direction == 1) {
+    bool ok = prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
+    if (ok) Wn = seg[t + W];
+  } else {
+    bool ok = prop_mode ? (x > 1 && x < W - 2) : (x > 0 && x < W);
+    if (ok) Wn = seg[t - 1];
+  }
+  if (Wn >= 0 && C != Wn) {
+    if (prop_mode && (F.P.prop[C] == F.P.prop[Wn])) return;
+    atomicMax(F.sm.pairs + 2 * C + 1, Wn);
+  }
+}
+
+__global__ void kb_merge_step0(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf || F.P.valid[k] == 0) return;
+  SmState sm = F.sm;
+  int fk = sm.pairs[2 * k + 1];
+  if (fk < 0) return;
+  const NigConsts& nc = F.nc;
+  int count_f = F.P.count[fk], count_k = F.P.count[k];
+  int count_fk = count_f + count_k;
+  sm.count_f[k] = count_fk;
+  for (int c = 0; c < 3; c++) {
+    double Qk = fx_to_double(F.sst.Q[3 * k + c]), Qf = fx_to_double(F.sst.Q[3 * fk + c]);
+    double Sk = fx_to_double(F.st.S[3 * k + c]), Sf = fx_to_double(F.st.S[3 * fk + c]);
+    double bn = nc.b0 + 0.5 * (Qk - (Sk * Sk) / (double)count_k);
+    double bf = nc.b0 + 0.5 * ((Qk + Qf) - ((Sf + Sk) * (Sf + Sk)) / (double)count_fk);
+    if (bn < 0) bn = 0.1;
+    if (bf < 0) bf = 0.1;
+    sm.b_n[3 * k + c] = bn;
+    sm.b_n_f[3 * k + c] = bf;
+  }
+  sm.has_bn[k] = 1;
+}
+
+__global__ void kb_merge_marg(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf || F.P.valid[k] == 0) return;
+  F.sm.marg[k] = nig_marg(F.nc, (double)F.P.count[k], F.sm.b_n + 3 * k);
+}
+
+__global__ void kb_merge_eval(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf || F.P.valid[k] == 0) return;
+  SmState sm = F.sm;
+  int fk = sm.pairs[2 * k + 1];
+  if (fk < 0 || F.P.valid[fk] == 0) return;
+  int count_k = F.P.count[k], count_fk = sm.count_f[k];
+  if (count_k < 1 || count_fk < 1) return;
+  double mfk = nig_marg(F.nc, (double)count_fk, sm.b_n_f + 3 * k);
+  double h = mfk - (F.nc.log_alpha + sm.marg[k] + sm.marg[fk]) + (double)F.merge_alpha;
+  sm.hasting[k] = h;
+  if (h > -2.0) atomicMax(sm.winner + fk, k);
+}
+
+__global__ void kb_merge_flag(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  SmState sm = F.sm;
+  int w = sm.winner[k];
+  if (w >= 0) {
+    sm.merge[w] = 1;
+    sm.pairs[2 * k] = w;
+  }
+  sm.valid0[k] = F.P.valid[k];
+  sm.pc0[k] = F.P.prior_count[k];
+  sm.rank[k] = sm.pairs[2 * k + 1];  // target snapshot
+}
+
+__global__ void kb_merge_remove(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  SmState sm = F.sm;
+  Params P = F.P;
+  const int* tgt0 = sm.rank;
+  int s = tgt0[k];
+  if (s < 0) return;
+  int is_cycle = tgt0[s] == k;
+  if (sm.valid0[k] == 0 || sm.valid0[s] == 0) return;
+  int mk = sm.merge[k], msg = sm.merge[s];
+  if (!F.prop_mode) {
+    if (mk && (!msg || (is_cycle && k < s))) {
+      sm.remove[k] = 1;
+      P.valid[k] = 0;
+      P.prior_count[s] = sm.pc0[k] + sm.pc0[s];
+    }
+  } else {
+    int is_prop_k = P.prop[k];
+    int cond_c = is_cycle && is_prop_k;
+    if (mk && (!msg || cond_c)) {
+      int victim = k, keeper = s;
+      if (is_prop_k) {
+        sm.pairs[2 * s + 1] = k;
+        victim = s;
+        keeper = k;
+      }
+      sm.remove[victim] = 1;
+      P.valid[victim] = 0;
+      P.prior_count[keeper] = sm.pc0[victim] + sm.pc0[keeper];
+    }
+  }
+}
+
+__global__ void kb_merge_apply(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t >= F.npix) return;
+  int* seg = F.seg[cur];
+  int k = seg[t];
+  if (k >= 0 && k < F.nbuf && F.sm.remove[k]) seg[t] = F.sm.pairs[2 * k + 1];
+}
+
+void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
+  kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
+  kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
+  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 2);
+  kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_merge_marg<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_merge_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_merge_flag<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_merge_remove<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_merge_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+}
+
+// ---------------------------------------------------------------------------
+// relabel (relabel.cu:85-409), rules R7, R8.  The only candidates the
+// reference's rule can pick for a living label are the previous-frame ids
+// that are dead now, and the label itself; the search scans exactly those.
+// ---------------------------------------------------------------------------
+__global__ void kb_relabel_prep(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  F.alive[k] = 0;
+  F.target[k] = -1;
+  F.want_new[k] = 0;
+  F.src_of[k] = -1;
+}
+
+__global__ void kb_mark_alive(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t >= F.npix) return;
+  int k = F.seg[cur][t];
+  if (k >= 0 && k < F.nbuf) F.alive[k] = 1;
+}
+
+__global__ void kb_relabel_lists(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const int nspix = *F.max_spix + 1;
+  if (k < F.nspix_prev && !F.alive[k]) F.dead[atomicAdd(F.counter, 1)] = k;
+  if (k < nspix && F.alive[k]) {
+    int pos = atomicAdd(b.work_n, 1);
+    if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
+  }
+}
+
+// persistent waves: one living label per wave at a time
+__global__ void __launch_bounds__(256) kb_relabel_search(Batch b) {
+  const int lane = threadIdx.x & 63;
+  const int n_items = min(b.work_n[0], b.work_cap);
+  while (true) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(b.work_n + 1, 1);
+    it = __shfl(it, 0);
+    if (it >= n_items) break;
+    const int wv = b.work[it];
+    const int f = wv >> 24, s = wv & 0xffffff;
+    const FrameDesc& F = b.fr[f];
+    const float* mu = F.P.mu_app + 3 * s;
+    const float m0 = mu[0], m1 = mu[1], m2 = mu[2];
+    const float* pm = F.prev_prior_mu_app;
+    const int ndead = *F.counter;
+    float best = 10000.0f;
+    int cand = -1;
+    float ss = 0.0f;
+    // candidates: the dead previous ids, then the label itself
+    for (int j = lane; j <= ndead; j += 64) {
+      int c;
+      if (j < ndead) {
+        c = F.dead[j];
+      } else {
+        if (s >= F.nspix_prev) continue;
+        c = s;
+      }
+      float dx = m0 - pm[3 * c];
+      float dy = m1 - pm[3 * c + 1];
+      float dz = m2 - pm[3 * c + 2];
+      float delta = (dx * dx + dy * dy + dz * dz);
+      if (c == s) ss = delta;
+      if (delta < best || (delta == best && cand >= 0 && c < cand)) {
+        best = delta;
+        cand = c;
+      }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+      float b2 = __shfl_xor(best, d);
+      int c2 = __shfl_xor(cand, d);
+      bool take = (c2 >= 0) && (cand < 0 || b2 < best || (b2 == best && c2 < cand));
+      if (take) {
+        best = b2;
+        cand = c2;
+      }
+    }
+    for (int d = 32; d >= 1; d >>= 1) ss = fmaxf(ss, __shfl_xor(ss, d));
+    if (lane == 0) {
+      const int nspix = *F.max_spix + 1;
+      bool revive = (cand >= 0) && (best < F.thresh_relabel) && (cand < nspix) && (cand != s);
+      F.target[s] = revive ? cand : -1;
+      F.want_new[s] = (!revive && ss > F.thresh_new) ? 1 : 0;
+    }
+  }
+}
+
+__global__ void kb_relabel_assign(Batch b) {
+  LAB_PROLOGUE
+  const int nspix = *F.max_spix + 1;
+  if (k >= nspix || k >= F.nbuf) return;
+  if (F.want_new[k]) {
+    int id = nspix + F.rank[k];
+    F.target[k] = id < F.nbuf ? id : -1;
+  }
+  int t = F.target[k];
+  if (t >= 0) atomicMax(F.src_of + t, k);
+}
+
+__global__ void kb_relabel_params(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const int s = F.src_of[k];
+  if (s < 0) return;
+  Params P = F.P;
+  P.count[k] = P.count[s];
+  P.icount[k] = P.icount[s];
+  if (k >= F.nspix_prev) {
+    P.prop[k] = 0;
+    P.prior_count[k] = P.prior_count[s];
+    for (int c = 0; c < 3; c++) {
+      P.prior_sigma_shape[3 * k + c] = P.prior_sigma_shape[3 * s + c];
+      P.sample_sigma_shape[3 * k + c] = P.sample_sigma_shape[3 * s + c];
+      P.prior_mu_app[3 * k + c] = P.prior_mu_app[3 * s + c];
+      P.prior_icov[3 * k + c] = P.prior_icov[3 * s + c];
+    }
+  }
+}
+
+__global__ void kb_relabel_valid(Batch b) {
+  LAB_PROLOGUE
+  if (k > *F.max_spix || k >= F.nbuf) return;
+  int t = F.target[k];
+  if (t >= 0) {
+    F.P.valid[k] = 0;
+    F.P.valid[t] = 1;
+  }
+}
+
+__global__ void kb_relabel_apply(Batch b, int cur) {
+  PIX_PROLOGUE
+  if (t >= F.npix) return;
+  int* seg = F.seg[cur];
+  int L = seg[t];
+  if (L >= 0 && L < F.nbuf) {
+    int tg = F.target[L];
+    if (tg >= 0) seg[t] = tg;
+  }
+}
+
+__global__ void kb_relabel_max(Batch b) {
+  int f = threadIdx.x;
+  if (f >= b.nf) return;
+  const FrameDesc& F = b.fr[f];
+  int m = *F.max_spix + *F.total;
+  if (m > F.nbuf - 1) m = F.nbuf - 1 > *F.max_spix ? F.nbuf - 1 : *F.max_spix;
+  *F.max_spix = m;
+}
+
+void b_relabel(const Batch& b, int cur, hipStream_t s) {
+  kb_relabel_prep<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_mark_alive<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_reset_work<<<1, 64, 0, s>>>(b);
+  kb_relabel_lists<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_relabel_search<<<512, 256, 0, s>>>(b);
+  kb_scan<<<b.nf, 1024, 0, s>>>(b, 1);
+  kb_relabel_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_relabel_params<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_relabel_valid<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_relabel_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_relabel_max<<<1, 64, 0, s>>>(b);
+}
+
+}  // namespace stsp

@@ -365,140 +365,4 @@ void launch_prop_params_init(Params P, InParams in, int nspix, int nbuf, int sp_
   k_mark_active_px<<<div_up(npix, kBlock), kBlock, 0, s>>>(filled, npix, P, nbuf);
 }
 
-// ---------------------------------------------------------------------------
-// relabel (relabel.cu:85-276), rules R7, R8.  One wave per living label scans
-// the previous-frame candidates; (delta, id) is reduced lexicographically.
-// ---------------------------------------------------------------------------
-__global__ void k_relabel_search(Params P, const float* __restrict__ prev_prior_mu_app,
-                                 const int* __restrict__ alive, int nspix_prev,
-                                 const int* max_spix, float thresh_replace, float thresh_new,
-                                 int* target, int* want_new) {
-  int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
-  int nspix = *max_spix + 1;
-  int s = wave;
-  if (s >= nspix) return;
-  if (!alive[s]) {
-    if (lane == 0) {
-      target[s] = -1;
-      want_new[s] = 0;
-    }
-    return;
-  }
-  float m0 = P.mu_app[3 * s], m1 = P.mu_app[3 * s + 1], m2 = P.mu_app[3 * s + 2];
-  float best = 10000.0f;
-  int cand = -1;
-  float ss = 0.0f;
-  for (int c = lane; c < nspix_prev; c += 64) {
-    bool same = (c == s);
-    if (alive[c] && !same) continue;
-    float dx = m0 - prev_prior_mu_app[3 * c];
-    float dy = m1 - prev_prior_mu_app[3 * c + 1];
-    float dz = m2 - prev_prior_mu_app[3 * c + 2];
-    float delta = (dx * dx + dy * dy + dz * dz);
-    if (same) ss = delta;
-    if (delta < best) {
-      best = delta;
-      cand = c;
-    }
-  }
-  // lexicographic (delta, id) minimum over the wave; "none" sorts last
-  for (int d = 32; d >= 1; d >>= 1) {
-    float b2 = __shfl_xor(best, d);
-    int c2 = __shfl_xor(cand, d);
-    bool take = (c2 >= 0) && (cand < 0 || b2 < best || (b2 == best && c2 < cand));
-    if (take) {
-      best = b2;
-      cand = c2;
-    }
-  }
-  for (int d = 32; d >= 1; d >>= 1) ss = fmaxf(ss, __shfl_xor(ss, d));  // only one lane set it
-  if (lane == 0) {
-    bool revive = (cand >= 0) && (best < thresh_replace) && (cand < nspix) && (cand != s);
-    target[s] = revive ? cand : -1;
-    want_new[s] = (!revive && ss > thresh_new) ? 1 : 0;
-  }
-}
-
-__global__ void k_relabel_assign(int* target, const int* want_new, const int* rank,
-                                 const int* max_spix, int nbuf, int* src_of, int n) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  int nspix = *max_spix + 1;
-  if (s >= nspix || s >= n) return;
-  if (want_new[s]) {
-    int id = nspix + rank[s];
-    target[s] = id < nbuf ? id : -1;
-  }
-  int t = target[s];
-  if (t >= 0) atomicMax(src_of + t, s);
-}
-
-__global__ void k_relabel_params(Params P, const int* src_of, int nbuf, int nspix_prev) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nbuf) return;
-  int s = src_of[n];
-  if (s < 0) return;
-  P.count[n] = P.count[s];
-  P.icount[n] = P.icount[s];
-  if (n >= nspix_prev) {
-    P.prop[n] = 0;
-    P.prior_count[n] = P.prior_count[s];
-    for (int c = 0; c < 3; c++) {
-      P.prior_sigma_shape[3 * n + c] = P.prior_sigma_shape[3 * s + c];
-      P.sample_sigma_shape[3 * n + c] = P.sample_sigma_shape[3 * s + c];
-      P.prior_mu_app[3 * n + c] = P.prior_mu_app[3 * s + c];
-      P.prior_icov[3 * n + c] = P.prior_icov[3 * s + c];
-    }
-  }
-}
-
-__global__ void k_relabel_valid(Params P, const int* target, const int* max_spix, int n) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > *max_spix || s >= n) return;
-  int t = target[s];
-  if (t >= 0) {
-    P.valid[s] = 0;
-    P.valid[t] = 1;
-  }
-}
-
-__global__ void k_relabel_apply(int* seg, const int* target, int npix, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  int L = seg[i];
-  if (L >= 0 && L < n) {
-    int t = target[L];
-    if (t >= 0) seg[i] = t;
-  }
-}
-
-__global__ void k_relabel_max(int* max_spix, const int* total, int nbuf) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int m = *max_spix + *total;
-    if (m > nbuf - 1) m = nbuf - 1 > *max_spix ? nbuf - 1 : *max_spix;
-    *max_spix = m;
-  }
-}
-
-void launch_relabel(int* seg, int npix, Params P, int nbuf, const float* prev_prior_mu_app,
-                    int nspix_prev, int* max_spix, float thresh_replace, float thresh_new,
-                    int* alive, int* target, int* want_new, int* rank, int* src_of, int* total,
-                    hipStream_t s) {
-  launch_fill(alive, 0, nbuf, s);
-  launch_mark_alive(seg, npix, alive, nbuf, s);
-  launch_fill(target, -1, nbuf, s);
-  launch_fill(want_new, 0, nbuf, s);
-  launch_fill(src_of, -1, nbuf, s);
-  k_relabel_search<<<div_up((long long)nbuf * 64, kBlock), kBlock, 0, s>>>(
-      P, prev_prior_mu_app, alive, nspix_prev, max_spix, thresh_replace, thresh_new, target,
-      want_new);
-  launch_scan(want_new, rank, nbuf, total, s);
-  k_relabel_assign<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(target, want_new, rank, max_spix, nbuf,
-                                                           src_of, nbuf);
-  k_relabel_params<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, src_of, nbuf, nspix_prev);
-  k_relabel_valid<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, target, max_spix, nbuf);
-  k_relabel_apply<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, target, npix, nbuf);
-  k_relabel_max<<<1, 1, 0, s>>>(max_spix, total, nbuf);
-}
-
 }  // namespace stsp

@@ -1,0 +1,107 @@
+"""The batched (served) form: each frame's result in a batch equals its
+result alone and the oracle's; hipGraph replays are checked on every replay
+with eager work between them."""
+import numpy as np
+import pytest
+import torch
+
+from st_spix_amd import core, synth
+
+from test_gpu_parity import assert_labels, assert_params, cuda, orc_cfg
+
+pytestmark = pytest.mark.gpu
+
+STREAMS = [(48, 64, 8, 21), (40, 56, 8, 22), (64, 80, 10, 23), (33, 50, 7, 24)]
+
+
+def make_videos(T):
+    return [synth.voronoi_video(T, H, W, nregions=8, vmax=3.0, seed=seed)
+            for H, W, sp, seed in STREAMS]
+
+
+def run_oracle(orc, vids, kw):
+    out = []
+    for (H, W, sp, _), (vid, flows) in zip(STREAMS, vids):
+        cfg = orc_cfg(orc, sp_size=sp, **kw)
+        seg, p = orc.bass(vid[0], cfg)
+        frames = [(seg, {k: v.copy() for k, v in p.items()})]
+        for t in range(1, vid.shape[0]):
+            seg, p, _ = orc.prop(vid[t], flows[t - 1], seg, p, cfg)
+            frames.append((seg, {k: v.copy() for k, v in p.items()}))
+        out.append(frames)
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_prop_batch_matches_oracle(orc, use_graph):
+    T = 4
+    kw = dict(niters=12, potts=10.0, prop_niters=10)
+    vids = make_videos(T)
+    ref = run_oracle(orc, vids, kw)
+    ctxs, state = [], []
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        for (H, W, sp, _), (vid, flows) in zip(STREAMS, vids):
+            cfg = core.make_config(sp_size=sp, **kw)
+            ctx = core.Context(H, W, cfg)
+            spix, params = core.bass_frame(cuda(vid[0]), cfg, ctx=ctx)
+            ctxs.append(ctx)
+            state.append((spix, params))
+        runner = core.BatchRunner(8, use_graph=use_graph)
+        for t in range(1, T):
+            items = [(ctxs[i], cuda(vids[i][0][t]), cuda(vids[i][1][t - 1]), state[i][0],
+                      state[i][1]) for i in range(len(STREAMS))]
+            outs = runner.prop(items, stream=side)
+            side.synchronize()
+            # unrelated eager work between graph replays
+            junk = torch.randn(1 << 16, device="cuda").sum()
+            torch.cuda.synchronize()
+            del junk
+            for i, (spix, params, _) in enumerate(outs):
+                assert_labels(spix, ref[i][t][0], f"stream{i} frame{t}")
+                assert_params(params, ref[i][t][1], f"stream{i} frame{t}")
+                state[i] = (spix, params)
+    runner.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_server_alone_vs_in_mix():
+    """Each stream's labels are the same when it is served alone and when it
+    shares batches with other streams."""
+    from st_spix_amd import serve
+    T = 3
+    vids = make_videos(T)
+    cfgs = [core.make_config(niters=10, sp_size=sp, potts=10.0, prop_niters=8)
+            for (_, _, sp, _) in STREAMS]
+
+    def serve_streams(idx):
+        srv = serve.Server(hbm_budget_gb=8, max_batch=8)
+        sids = [srv.open_stream(STREAMS[i][0], STREAMS[i][1], cfgs[i]) for i in idx]
+        labels = {i: [] for i in idx}
+        for t in range(T):
+            for i, sid in zip(idx, sids):
+                vid, flows = vids[i]
+                srv.submit(sid, cuda(vid[t]), cuda(flows[t - 1]) if t > 0 else None)
+            while srv.pending():
+                srv.step()
+            for i, sid in zip(idx, sids):
+                labels[i].append(srv.labels(sid).cpu().numpy())
+        srv.close()
+        return labels
+
+    mix = serve_streams(list(range(len(STREAMS))))
+    for i in range(len(STREAMS)):
+        alone = serve_streams([i])
+        for t in range(T):
+            np.testing.assert_array_equal(mix[i][t], alone[i][t])
+
+
+def test_server_admission_rejects():
+    from st_spix_amd import serve
+    srv = serve.Server(hbm_budget_gb=0.05, max_batch=4)
+    srv.open_stream(64, 64)
+    with pytest.raises(serve.AdmissionError):
+        for _ in range(100):
+            srv.open_stream(1080, 1920)
+    srv.close()
