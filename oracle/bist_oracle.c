@@ -45,6 +45,9 @@
  *      initial min-size 100000 drops them, shift_labels.cu:213-216).
  *  R15 logs go through the deterministic stsp_det_log (FP64); the float
  *      narrowing of logdet_sigma_shape follows update_seg.cu:223.
+ *  R16 a split whose new id max_spix+1+rank would not fit in the nbuf slots
+ *      is dropped (the reference writes past its buffer,
+ *      split_merge_prop.cu:3111-3144); relabel never spawns ids >= nbuf.
  */
 #include "bist_oracle.h"
 
@@ -717,6 +720,10 @@ static int split_step(const float* img, int* seg, const int* shifted, osp* P,
   int next = max_spix;
   for (int k = 0; k < nbuf; k++) {
     if (!acc[k]) continue;
+    if (next + 1 >= nbuf) { /* no slot left for the new label: dropped */
+      acc[k] = 0;
+      continue;
+    }
     int s = ++next;
     w->pairs[2 * k] = s;
     if (!prop_mode) {

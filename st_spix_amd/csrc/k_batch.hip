@@ -711,6 +711,10 @@ __global__ void kb_split_assign(Batch b) {
   if (ms > F.nbuf / 2 || !F.sm.accept[k]) return;
   Params P = F.P;
   const int s = ms + 1 + F.sm.rank[k];
+  if (s >= F.nbuf) {  // no slot left for the new label: the split is dropped
+    F.sm.accept[k] = 0;
+    return;
+  }
   F.sm.pairs[2 * k] = s;
   if (!F.prop_mode) {
     float pc = (float)fmax(P.prior_count[k] / 2.0, 8.0);
@@ -764,8 +768,9 @@ __global__ void kb_add_max(Batch b) {
   int f = threadIdx.x;
   if (f >= b.nf) return;
   const FrameDesc& F = b.fr[f];
-  if (*F.max_spix > F.nbuf / 2) return;
-  *F.max_spix += *F.total;
+  const int ms = *F.max_spix;
+  if (ms > F.nbuf / 2) return;
+  *F.max_spix = ms + min(*F.total, F.nbuf - 1 - ms);
 }
 
 void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
