@@ -44,6 +44,14 @@ def parse():
     return ap.parse_args()
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr; stdout carries only the JSON line."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def pct(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -187,8 +195,12 @@ def main():
 
     from st_spix_amd import core, synth
     streams = make_streams(args, torch, synth)
+    log(f"{len(streams)} synthetic streams resident in HBM")
     served = run_served(args, streams, torch, core)
+    log(f"served: {served['elapsed']:.3f} s for {args.steps} steps")
     plain = None if args.no_plain else run_plain(args, streams, torch, core)
+    if plain is not None:
+        log(f"plain loop: {plain['elapsed']:.3f} s")
     nframes = len(streams) * args.steps
     fps = nframes / served["elapsed"]
     mpx = sum(s["H"] * s["W"] for s in streams) * args.steps / served["elapsed"] / 1e6
@@ -221,6 +233,7 @@ def main():
                          "p99_ms": round(pct(plain["lat"], 99) * 1e3, 3)}
         line["served_over_plain"] = round(fps / pfps, 3)
     if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line), flush=True)
 

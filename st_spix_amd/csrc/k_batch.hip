@@ -566,12 +566,12 @@ __global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
   __shared__ uint8_t m[kBfsCapB];
   __shared__ int item, changed;
   const int tid = threadIdx.x;
-  const int n_items = min(b.work_n[0], b.work_cap);
+  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[0], b.work_cap));
   while (true) {
     __syncthreads();
     if (tid == 0) item = atomicAdd(b.work_n + 1, 1);
     __syncthreads();
-    const int it = item;
+    const int it = __builtin_amdgcn_readfirstlane(item);  // block-uniform, scalar branch
     if (it >= n_items) break;
     const int wv = b.work[it];
     const int f = wv >> 24, k = wv & 0xffffff;
@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
         }
         if (ch) changed = 1;
         __syncthreads();
-        const int go = changed;
+        const int go = __builtin_amdgcn_readfirstlane(changed);
         __syncthreads();
         if (!go) break;
       }
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
       }
       if (ch) changed = 1;
       __syncthreads();
-      const int go = changed;
+      const int go = __builtin_amdgcn_readfirstlane(changed);
       __syncthreads();
       if (!go) break;
     }
@@ -971,14 +971,19 @@ __global__ void kb_relabel_lists(Batch b) {
   }
 }
 
-// persistent waves: one living label per wave at a time
+// Persistent waves: one living label per wave at a time.  The loop has no
+// lane-dependent branch: every lane takes part in the atomic (only lane 0
+// adds 1) and the item index is made wave-uniform with readfirstlane, so the
+// loop exit is a scalar branch.  (With `if (lane == 0) it = atomicAdd(..)`
+// plus a lane-0 store at the end of the body, the compiler threads the
+// lane-0 paths across the back-edge into a loop nest in which lanes 1..63
+// re-read a reset index from lane 0 and never leave.)
 __global__ void __launch_bounds__(256) kb_relabel_search(Batch b) {
   const int lane = threadIdx.x & 63;
-  const int n_items = min(b.work_n[0], b.work_cap);
+  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[0], b.work_cap));
   while (true) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(b.work_n + 1, 1);
-    it = __shfl(it, 0);
+    const int mine = atomicAdd(b.work_n + 1, lane == 0 ? 1 : 0);
+    const int it = __builtin_amdgcn_readfirstlane(mine);
     if (it >= n_items) break;
     const int wv = b.work[it];
     const int f = wv >> 24, s = wv & 0xffffff;
@@ -1019,12 +1024,11 @@ __global__ void __launch_bounds__(256) kb_relabel_search(Batch b) {
       }
     }
     for (int d = 32; d >= 1; d >>= 1) ss = fmaxf(ss, __shfl_xor(ss, d));
-    if (lane == 0) {
-      const int nspix = *F.max_spix + 1;
-      bool revive = (cand >= 0) && (best < F.thresh_relabel) && (cand < nspix) && (cand != s);
-      F.target[s] = revive ? cand : -1;
-      F.want_new[s] = (!revive && ss > F.thresh_new) ? 1 : 0;
-    }
+    // every lane holds the reduced values; all lanes store the same words
+    const int nspix = *F.max_spix + 1;
+    const bool revive = (cand >= 0) && (best < F.thresh_relabel) && (cand < nspix) && (cand != s);
+    F.target[s] = revive ? cand : -1;
+    F.want_new[s] = (!revive && ss > F.thresh_new) ? 1 : 0;
   }
 }
 
