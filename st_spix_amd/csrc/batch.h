@@ -58,8 +58,8 @@ struct Batch {
   BatchOff off;
   int nf;
   int pix_blocks, lab_blocks, sm_blocks, tile_blocks[3];
-  int* work;      // work list (frame << 24 | label), capacity work_cap
-  int* work_n;    // [0] length, [1] cursor
+  int* work;      // two work lists (frame << 24 | label) of work_cap entries each
+  int* work_n;    // [0] length, [1] cursor of list 0; [2], [3] of list 1
   int work_cap;
 };
 

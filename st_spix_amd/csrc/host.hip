@@ -441,7 +441,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
       HIP_TRY(hipStreamSynchronize(s));
       HIP_TRY(hipFree(bh->d_work));
     }
-    HIP_TRY(hipMalloc((void**)&bh->d_work, sizeof(int) * (size_t)work));
+    HIP_TRY(hipMalloc((void**)&bh->d_work, 2 * sizeof(int) * (size_t)work));  // two lists
     bh->work_cap = work;
     invalidate_graph(bh);
   }
@@ -879,6 +879,7 @@ int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
     if (rc) return rc;
   }
   HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) tflush(ctxs[i]);  // timed launches are recorded on group leads
   for (int i = 0; i < n; i++) {
     int rc = epilogue_finish(ctxs[i], spix_out[i], &outs[i], &nspix_out[i], s);
     if (rc) return rc;

@@ -488,10 +488,7 @@ __global__ void __launch_bounds__(1024) kb_scan(Batch b, int which) {
 }
 
 __global__ void kb_reset_work(Batch b) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    b.work_n[0] = 0;
-    b.work_n[1] = 0;
-  }
+  if (threadIdx.x < 4 && blockIdx.x == 0) b.work_n[threadIdx.x] = 0;
   if (blockIdx.x == 0 && threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
 }
 
@@ -532,13 +529,22 @@ __global__ void kb_copy_halves(Batch b, int cur) {
   if (t < F.npix) F.halves[t] = F.seg[cur][t];
 }
 
-// labels that take part in the split: valid, non-empty, split not skipped
+// Labels that take part in the split (valid, non-empty, split not skipped),
+// sorted by the size of their bounding box: boxes up to kWaveBox x kWaveBox
+// go to the wave list work[0..), larger ones to the block list
+// work[work_cap..).
+constexpr int kWaveBox = 128;
+
 __global__ void kb_split_list(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
   if (*F.max_spix > F.nbuf / 2 || F.P.valid[k] == 0 || F.P.count[k] <= 0) return;
-  int pos = atomicAdd(b.work_n, 1);
-  if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
+  const int* bb = F.st.bbox + 4 * k;
+  const int bw = bb[1] - bb[0] + 1, bh = bb[3] - bb[2] + 1;
+  if (bw < 1 || bh < 1) return;
+  const bool small = bw <= kWaveBox && bh <= kWaveBox;
+  const int pos = atomicAdd(b.work_n + (small ? 0 : 2), 1);
+  if (pos < b.work_cap) b.work[(small ? 0 : b.work_cap) + pos] = (f << 24) | k;
 }
 
 __device__ __forceinline__ int seed_of(const FrameDesc& F, const int* seg, int k, int dir) {
@@ -555,84 +561,268 @@ __device__ __forceinline__ int seed_of(const FrameDesc& F, const int* seg, int k
   return seg[ind] == k ? ind : -1;
 }
 
-constexpr int kBfsCapB = 8192;
+// ---------------------------------------------------------------------------
+// Two-seed split proposal (calc_split_candidate, split_merge.cu:757-800, and
+// calc_seg_split 847-860) as a bit-parallel BFS.  The label's pixels inside
+// its bounding box form a bitmask M (one 64-bit word per 64 columns of a
+// row).  Level by level the visited sets grow as
+//   N = (V<<1 | V>>1 | row above | row below) & M & ~V
+// which reaches exactly the pixels at 4-connected distance L at level L.
+// The reference marks a pixel for the new label when d1 > d2, with d = -1
+// for pixels a seed never reaches.  That is: reached from seed 1, and not
+// reached from seed 2 at a level no earlier than seed 1.  NK collects the
+// pixels that seed 2 reaches while seed 1 already has them (d2 >= d1), so
+// the new label takes V1 & ~NK.
+// ---------------------------------------------------------------------------
+template <int RC, int WD>
+__device__ __forceinline__ void bits_expand(const uint64_t (&V)[RC][WD],
+                                            const uint64_t (&M)[RC][WD], uint64_t (&N)[RC][WD],
+                                            int lane) {
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      const uint64_t v = V[c][w];
+      uint64_t h = (v << 1) | (v >> 1);
+      if (w > 0) h |= V[c][w - 1] >> 63;
+      if (w + 1 < WD) h |= V[c][w + 1] << 63;
+      uint64_t up = __shfl_up(v, 1);
+      const uint64_t last_prev = c > 0 ? __shfl(V[c > 0 ? c - 1 : 0][w], 63) : 0ull;
+      up = lane == 0 ? last_prev : up;
+      uint64_t dn = __shfl_down(v, 1);
+      const uint64_t first_next = c + 1 < RC ? __shfl(V[c + 1 < RC ? c + 1 : c][w], 0) : 0ull;
+      dn = lane == 63 ? first_next : dn;
+      N[c][w] = (h | up | dn) & M[c][w] & ~v;
+    }
+  }
+}
 
-// Persistent workgroups pull labels from the work list and run the two-seed
-// level-synchronous BFS of calc_split_candidate (split_merge.cu:757-800)
-// inside the label's bounding box (LDS; global maps for oversized boxes),
-// then mark the far half (calc_seg_split 847-860).
-__global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
-  __shared__ short d1[kBfsCapB], d2[kBfsCapB];
-  __shared__ uint8_t m[kBfsCapB];
-  __shared__ int item, changed;
-  const int tid = threadIdx.x;
+// One wave: rows 64c + lane of a box of at most 64*RC rows and 64*WD columns
+// live in registers.
+template <int RC, int WD>
+__device__ __forceinline__ void bfs_wave(const int* __restrict__ seg, int W, int k, int bx0,
+                                         int by0, int bw, int bh, int s1, int s2, int k2,
+                                         int* __restrict__ halves) {
+  const int lane = threadIdx.x & 63;
+  uint64_t M[RC][WD], V1[RC][WD], V2[RC][WD], NK[RC][WD];
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      M[c][w] = V1[c][w] = V2[c][w] = NK[c][w] = 0;
+      const int col = 64 * w + lane;
+#pragma unroll 8
+      for (int rr = 0; rr < 64; rr++) {
+        const int row = 64 * c + rr;
+        bool bit = false;
+        if (row < bh && col < bw) bit = seg[(by0 + row) * W + bx0 + col] == k;
+        const uint64_t word = __ballot(bit);
+        M[c][w] = lane == rr ? word : M[c][w];
+      }
+    }
+  }
+  // seeds (level 1)
+  const int sy1 = s1 >= 0 ? s1 / W - by0 : -1, sx1 = s1 >= 0 ? s1 % W - bx0 : 0;
+  const int sy2 = s2 >= 0 ? s2 / W - by0 : -1, sx2 = s2 >= 0 ? s2 % W - bx0 : 0;
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      if (sy1 == 64 * c + lane && (sx1 >> 6) == w) V1[c][w] |= 1ull << (sx1 & 63);
+      if (sy2 == 64 * c + lane && (sx2 >> 6) == w) V2[c][w] |= 1ull << (sx2 & 63);
+    }
+  }
+  while (true) {
+    uint64_t N1[RC][WD], N2[RC][WD];
+    bits_expand<RC, WD>(V1, M, N1, lane);
+    bits_expand<RC, WD>(V2, M, N2, lane);
+    uint64_t grew = 0;
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        V1[c][w] |= N1[c][w];
+        NK[c][w] |= N2[c][w] & V1[c][w];
+        V2[c][w] |= N2[c][w];
+        grew |= N1[c][w] | N2[c][w];
+      }
+    }
+    if (__ballot(grew != 0) == 0) break;  // wave-uniform
+  }
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      const uint64_t K = V1[c][w] & ~NK[c][w];
+      const int col = 64 * w + lane;
+#pragma unroll 8
+      for (int rr = 0; rr < 64; rr++) {
+        const uint64_t word = __shfl(K, rr);
+        const int row = 64 * c + rr;
+        if (row < bh && col < bw && ((word >> lane) & 1ull))
+          halves[(by0 + row) * W + bx0 + col] = k2;
+      }
+    }
+  }
+}
+
+// Persistent waves over the wave list.  Loop control is wave-uniform
+// (readfirstlane) and free of lane-conditional branches.
+__global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int direction) {
+  const int lane = threadIdx.x & 63;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[0], b.work_cap));
   while (true) {
-    __syncthreads();
-    if (tid == 0) item = atomicAdd(b.work_n + 1, 1);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(item);  // block-uniform, scalar branch
+    const int mine = atomicAdd(b.work_n + 1, lane == 0 ? 1 : 0);
+    const int it = __builtin_amdgcn_readfirstlane(mine);
     if (it >= n_items) break;
-    const int wv = b.work[it];
+    const int wv = __builtin_amdgcn_readfirstlane(b.work[it]);
+    const int f = wv >> 24, k = wv & 0xffffff;
+    const FrameDesc& F = b.fr[f];
+    const int* seg = F.seg[cur];
+    const int* bb = F.st.bbox + 4 * k;
+    const int bx0 = __builtin_amdgcn_readfirstlane(bb[0]);
+    const int by0 = __builtin_amdgcn_readfirstlane(bb[2]);
+    const int bw = __builtin_amdgcn_readfirstlane(bb[1]) - bx0 + 1;
+    const int bh = __builtin_amdgcn_readfirstlane(bb[3]) - by0 + 1;
+    const int s1 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, direction));
+    const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
+    const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
+    const int W = F.W;
+    if (bh <= 64) {
+      if (bw <= 64) bfs_wave<1, 1>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
+      else bfs_wave<1, 2>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
+    } else {
+      if (bw <= 64) bfs_wave<2, 1>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
+      else bfs_wave<2, 2>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
+    }
+  }
+}
+
+// Persistent workgroups over the block list: bitsets in LDS (up to
+// kBlkWords words), level-synchronous with two barriers per level; boxes
+// beyond that use per-pixel distances in global memory.
+constexpr int kBlkWords = 2048;
+
+__global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direction) {
+  __shared__ uint64_t sM[kBlkWords], sV1[kBlkWords], sV2[kBlkWords], sNK[kBlkWords];
+  __shared__ int item, changed[2];
+  const int tid = threadIdx.x;
+  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
+  while (true) {
+    __syncthreads();
+    if (tid == 0) item = atomicAdd(b.work_n + 3, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(item);
+    if (it >= n_items) break;
+    const int wv = __builtin_amdgcn_readfirstlane(b.work[b.work_cap + it]);
     const int f = wv >> 24, k = wv & 0xffffff;
     const FrameDesc& F = b.fr[f];
     const int* seg = F.seg[cur];
     const int W = F.W;
-    const int* bbox = F.st.bbox;
-    const int bx0 = bbox[4 * k], bx1 = bbox[4 * k + 1], by0 = bbox[4 * k + 2], by1 = bbox[4 * k + 3];
-    if (bx1 < bx0 || by1 < by0) continue;
-    const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1, nc = bw * bh;
-    const int s1 = seed_of(F, seg, k, direction);
-    const int s2 = seed_of(F, seg, k, -direction);
-    const int k2 = k + *F.max_spix + 1;
+    const int* bb = F.st.bbox + 4 * k;
+    const int bx0 = __builtin_amdgcn_readfirstlane(bb[0]);
+    const int by0 = __builtin_amdgcn_readfirstlane(bb[2]);
+    const int bw = __builtin_amdgcn_readfirstlane(bb[1]) - bx0 + 1;
+    const int bh = __builtin_amdgcn_readfirstlane(bb[3]) - by0 + 1;
+    const int s1 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, direction));
+    const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
+    const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
     int* halves = F.halves;
-    if (nc <= kBfsCapB) {
-      for (int i = tid; i < nc; i += 256) {
-        int g = (by0 + i / bw) * W + bx0 + i % bw;
-        m[i] = seg[g] == k;
-        d1[i] = (g == s1) ? 1 : -1;
-        d2[i] = (g == s2) ? 1 : -1;
+    const int WD = (bw + 63) >> 6;
+    const int nw = bh * WD;
+    if (nw <= kBlkWords) {
+      // mask: each wave ballots one 64-column word per step
+      for (int p = tid; p < nw * 64; p += 256) {
+        const int r = p / (WD * 64), col = p - r * (WD * 64);
+        const bool bit = col < bw && seg[(by0 + r) * W + bx0 + col] == k;
+        const uint64_t word = __ballot(bit);
+        sM[p >> 6] = word;  // every lane writes the same word
+        sV1[p >> 6] = 0;
+        sV2[p >> 6] = 0;
+        sNK[p >> 6] = 0;
+      }
+      if (tid < 2) changed[tid] = 0;
+      __syncthreads();
+      if (tid == 0) {
+        if (s1 >= 0) {
+          const int y = s1 / W - by0, x = s1 % W - bx0;
+          sV1[y * WD + (x >> 6)] |= 1ull << (x & 63);
+        }
+        if (s2 >= 0) {
+          const int y = s2 / W - by0, x = s2 % W - bx0;
+          sV2[y * WD + (x >> 6)] |= 1ull << (x & 63);
+        }
       }
       __syncthreads();
-      for (int level = 1;; level++) {
-        if (tid == 0) changed = 0;
-        __syncthreads();
+      for (int level = 0;; level++) {
+        uint64_t n1[kBlkWords / 256], n2[kBlkWords / 256];
         int ch = 0;
-        for (int i = tid; i < nc; i += 256) {
-          if (!m[i]) continue;
-          bool u1 = d1[i] == -1, u2 = d2[i] == -1;
-          if (!u1 && !u2) continue;
-          int lx = i % bw, ly = i / bw;
-          bool h1 = false, h2 = false;
-          if (ly > 0 && m[i - bw]) { h1 |= d1[i - bw] == level; h2 |= d2[i - bw] == level; }
-          if (lx > 0 && m[i - 1]) { h1 |= d1[i - 1] == level; h2 |= d2[i - 1] == level; }
-          if (ly < bh - 1 && m[i + bw]) { h1 |= d1[i + bw] == level; h2 |= d2[i + bw] == level; }
-          if (lx < bw - 1 && m[i + 1]) { h1 |= d1[i + 1] == level; h2 |= d2[i + 1] == level; }
-          if (u1 && h1) { d1[i] = (short)(level + 1); ch = 1; }
-          if (u2 && h2) { d2[i] = (short)(level + 1); ch = 1; }
+#pragma unroll
+        for (int j = 0; j < kBlkWords / 256; j++) {
+          const int i = tid + 256 * j;
+          n1[j] = n2[j] = 0;
+          if (i < nw) {
+            const int r = i / WD, w = i - r * WD;
+            const uint64_t m = sM[i], a = sV1[i], c = sV2[i];
+            uint64_t e1 = (a << 1) | (a >> 1), e2 = (c << 1) | (c >> 1);
+            if (w > 0) {
+              e1 |= sV1[i - 1] >> 63;
+              e2 |= sV2[i - 1] >> 63;
+            }
+            if (w + 1 < WD) {
+              e1 |= sV1[i + 1] << 63;
+              e2 |= sV2[i + 1] << 63;
+            }
+            if (r > 0) {
+              e1 |= sV1[i - WD];
+              e2 |= sV2[i - WD];
+            }
+            if (r + 1 < bh) {
+              e1 |= sV1[i + WD];
+              e2 |= sV2[i + WD];
+            }
+            n1[j] = e1 & m & ~a;
+            n2[j] = e2 & m & ~c;
+            ch |= (n1[j] | n2[j]) != 0;
+          }
         }
-        if (ch) changed = 1;
         __syncthreads();
-        const int go = __builtin_amdgcn_readfirstlane(changed);
+#pragma unroll
+        for (int j = 0; j < kBlkWords / 256; j++) {
+          const int i = tid + 256 * j;
+          if (i < nw && (n1[j] | n2[j])) {
+            const uint64_t v1 = sV1[i] | n1[j];
+            sV1[i] = v1;
+            sNK[i] |= n2[j] & v1;
+            sV2[i] |= n2[j];
+          }
+        }
+        if (ch) changed[level & 1] = 1;
+        if (tid == 0) changed[(level + 1) & 1] = 0;
         __syncthreads();
+        const int go = __builtin_amdgcn_readfirstlane(changed[level & 1]);
         if (!go) break;
       }
-      for (int i = tid; i < nc; i += 256)
-        if (m[i] && d1[i] > d2[i]) halves[(by0 + i / bw) * W + bx0 + i % bw] = k2;
+      for (int p = tid; p < nw * 64; p += 256) {
+        const int r = p / (WD * 64), col = p - r * (WD * 64);
+        const uint64_t K = sV1[p >> 6] & ~sNK[p >> 6];
+        if (col < bw && ((K >> (p & 63)) & 1ull)) halves[(by0 + r) * W + bx0 + col] = k2;
+      }
       continue;
     }
+    // very large boxes: level-synchronous distances in global memory
     int* d1g = F.d1;
     int* d2g = F.d2;
+    const int nc = bw * bh;
     for (int i = tid; i < nc; i += 256) {
       int g = (by0 + i / bw) * W + bx0 + i % bw;
       if (seg[g] != k) continue;
       d1g[g] = (g == s1) ? 1 : -1;
       d2g[g] = (g == s2) ? 1 : -1;
     }
+    if (tid < 2) changed[tid] = 0;
     __syncthreads();
     for (int level = 1;; level++) {
-      if (tid == 0) changed = 0;
-      __syncthreads();
       int ch = 0;
       for (int i = tid; i < nc; i += 256) {
         int lx = i % bw, ly = i / bw, g = (by0 + ly) * W + bx0 + lx;
@@ -651,10 +841,10 @@ __global__ void __launch_bounds__(256) kb_bfs(Batch b, int cur, int direction) {
         if (u1 && h1) { d1g[g] = level + 1; ch = 1; }
         if (u2 && h2) { d2g[g] = level + 1; ch = 1; }
       }
-      if (ch) changed = 1;
+      if (ch) changed[level & 1] = 1;
+      if (tid == 0) changed[(level + 1) & 1] = 0;
       __syncthreads();
-      const int go = __builtin_amdgcn_readfirstlane(changed);
-      __syncthreads();
+      const int go = __builtin_amdgcn_readfirstlane(changed[level & 1]);
       if (!go) break;
     }
     for (int i = tid; i < nc; i += 256) {
@@ -778,7 +968,8 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_copy_halves<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_reset_work<<<1, 64, 0, s>>>(b);
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  kb_bfs<<<1024, 256, 0, s>>>(b, cur, direction);
+  kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
+  kb_bfs_block<<<256, 256, 0, s>>>(b, cur, direction);
   kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 1);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);

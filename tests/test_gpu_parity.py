@@ -77,6 +77,21 @@ def test_bass_parity_config0(orc):
     assert_params(params, p_r, "config0 params")
 
 
+@pytest.mark.parametrize("H,W,sp,niters,nreg,seed", [
+    (200, 260, 60, 12, 6, 1),    # boxes over 64 px: two-word / two-chunk wave BFS
+    (300, 420, 140, 12, 5, 2),   # boxes over 128 px: LDS bitset BFS
+    (700, 900, 500, 8, 3, 3)])   # boxes over 2048 words: global-memory BFS
+def test_split_bfs_box_sizes(orc, H, W, sp, niters, nreg, seed):
+    """Large superpixels route the split proposal through every BFS path."""
+    img = synth.voronoi_image(H, W, nreg, seed=seed)
+    kw = dict(niters=niters, sp_size=sp, potts=10.0)
+    spix, params = gpu_bass(img, **kw)
+    torch.cuda.synchronize()
+    seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
+    assert_labels(spix, seg_r, f"{H}x{W} sp{sp} labels")
+    assert_params(params, p_r, f"{H}x{W} sp{sp} params")
+
+
 @pytest.mark.parametrize("shape", [0, 1, 2])
 def test_sweep_shapes_bit_identical(orc, shape):
     """Every launch shape of the fused sweep (1, 2 or 4 sweeps per launch,
