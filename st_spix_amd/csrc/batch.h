@@ -51,13 +51,17 @@ struct BatchOff {
   int* lab;      // 256-label blocks over cap
   int* sm;       // 256-slot blocks over 2*cap+2
   int* tile[3];  // sweep tiles for each launch shape
+  int* st;       // statistics tiles (kStatTile x kStatTile pixels)
 };
+
+constexpr int kOffArrays = 7;  // pix, lab, sm, tile[3], st
+constexpr int kStatTile = 32;
 
 struct Batch {
   FrameDesc* fr;  // device table
   BatchOff off;
   int nf;
-  int pix_blocks, lab_blocks, sm_blocks, tile_blocks[3];
+  int pix_blocks, lab_blocks, sm_blocks, tile_blocks[3], st_blocks;
   int* work;      // two work lists (frame << 24 | label) of work_cap entries each
   int* work_n;    // [0] length, [1] cursor of list 0; [2], [3] of list 1
   int work_cap;
@@ -67,6 +71,10 @@ struct Batch {
 inline int tiles_for_shape(int shape, int H, int W) {
   static const int tw[3] = {32, 32, 64}, th[3] = {16, 32, 32};
   return ((W + tw[shape] - 1) / tw[shape]) * ((H + th[shape] - 1) / th[shape]);
+}
+
+inline int stat_tiles(int H, int W) {
+  return ((W + kStatTile - 1) / kStatTile) * ((H + kStatTile - 1) / kStatTile);
 }
 
 // refinement-loop launches over a batch (k_batch.hip)

@@ -101,6 +101,59 @@ __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
   atomicAdd((unsigned long long*)p, (unsigned long long)v);
 }
 
+// Wave-wide reductions by DPP, result in lane 63.  Row shifts by 1, 2, 4, 8
+// build prefix sums inside each 16-lane row; row_bcast:15 (rows 1, 3) and
+// row_bcast:31 (rows 2, 3) carry the lower rows into the last lane.  Lanes a
+// move cannot source keep the identity.  All 64 lanes must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_take(int v, int identity) {
+  return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int64_t dpp_take64(int64_t v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(uint64_t)v, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)((uint64_t)v >> 32), CTRL, ROW_MASK, 0xf, false);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ __forceinline__ int wave_sum_to63(int v) {
+  v += dpp_take<0x111, 0xf>(v, 0);
+  v += dpp_take<0x112, 0xf>(v, 0);
+  v += dpp_take<0x114, 0xf>(v, 0);
+  v += dpp_take<0x118, 0xf>(v, 0);
+  v += dpp_take<0x142, 0xa>(v, 0);
+  v += dpp_take<0x143, 0xc>(v, 0);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_sum64_to63(int64_t v) {
+  v += dpp_take64<0x111, 0xf>(v);
+  v += dpp_take64<0x112, 0xf>(v);
+  v += dpp_take64<0x114, 0xf>(v);
+  v += dpp_take64<0x118, 0xf>(v);
+  v += dpp_take64<0x142, 0xa>(v);
+  v += dpp_take64<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ int wave_min_to63(int v) {
+  const int id = 0x7fffffff;
+  v = min(v, dpp_take<0x111, 0xf>(v, id));
+  v = min(v, dpp_take<0x112, 0xf>(v, id));
+  v = min(v, dpp_take<0x114, 0xf>(v, id));
+  v = min(v, dpp_take<0x118, 0xf>(v, id));
+  v = min(v, dpp_take<0x142, 0xa>(v, id));
+  v = min(v, dpp_take<0x143, 0xc>(v, id));
+  return v;
+}
+__device__ __forceinline__ int wave_max_to63(int v) {
+  const int id = -0x7fffffff - 1;
+  v = max(v, dpp_take<0x111, 0xf>(v, id));
+  v = max(v, dpp_take<0x112, 0xf>(v, id));
+  v = max(v, dpp_take<0x114, 0xf>(v, id));
+  v = max(v, dpp_take<0x118, 0xf>(v, id));
+  v = max(v, dpp_take<0x142, 0xa>(v, id));
+  v = max(v, dpp_take<0x143, 0xc>(v, id));
+  return v;
+}
+
 // Runs of equal keys inside one wave: returns true on the run head (lowest
 // lane of the run) and sets run_end to the first lane after the run.
 __device__ __forceinline__ bool wave_run(int key, int& run_end) {

@@ -98,7 +98,7 @@ struct KTimer {
 struct stsp_batch {
   int maxf = 0;
   FrameDesc* d_fr = nullptr;
-  int* d_off = nullptr;  // 6 arrays of maxf+1
+  int* d_off = nullptr;  // kOffArrays arrays of maxf+1
   FrameDesc* h_fr = nullptr;
   int* h_off = nullptr;
   int* d_work = nullptr;
@@ -296,10 +296,10 @@ static int ensure_prev(stsp_ctx* c, int n) {
 static int batch_alloc(stsp_batch* bh, int maxf) {
   bh->maxf = maxf;
   HIP_TRY(hipMalloc((void**)&bh->d_fr, sizeof(FrameDesc) * maxf));
-  HIP_TRY(hipMalloc((void**)&bh->d_off, sizeof(int) * 6 * (maxf + 1)));
+  HIP_TRY(hipMalloc((void**)&bh->d_off, sizeof(int) * kOffArrays * (maxf + 1)));
   HIP_TRY(hipMalloc((void**)&bh->d_work_n, sizeof(int) * 4));
   HIP_TRY(hipHostMalloc((void**)&bh->h_fr, sizeof(FrameDesc) * maxf, hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void**)&bh->h_off, sizeof(int) * 6 * (maxf + 1), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_off, sizeof(int) * kOffArrays * (maxf + 1), hipHostMallocDefault));
   return STSP_OK;
 }
 
@@ -425,8 +425,9 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   int* o_lab = off + M;
   int* o_sm = off + 2 * M;
   int* o_t[3] = {off + 3 * M, off + 4 * M, off + 5 * M};
+  int* o_st = off + 6 * M;
   int work = 0;
-  o_pix[0] = o_lab[0] = o_sm[0] = o_t[0][0] = o_t[1][0] = o_t[2][0] = 0;
+  o_pix[0] = o_lab[0] = o_sm[0] = o_t[0][0] = o_t[1][0] = o_t[2][0] = o_st[0] = 0;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = ctxs[i];
     bh->h_fr[i] = make_desc(c);
@@ -434,6 +435,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     o_lab[i + 1] = o_lab[i] + div_up(c->cap, kBlock);
     o_sm[i + 1] = o_sm[i] + div_up(2 * c->cap + 2, kBlock);
     for (int sh = 0; sh < 3; sh++) o_t[sh][i + 1] = o_t[sh][i] + tiles_for_shape(sh, c->H, c->W);
+    o_st[i + 1] = o_st[i] + stat_tiles(c->H, c->W);
     work += c->cap;
   }
   if (work > bh->work_cap) {
@@ -446,7 +448,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     invalidate_graph(bh);
   }
   HIP_TRY(hipMemcpyAsync(bh->d_fr, bh->h_fr, sizeof(FrameDesc) * n, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(bh->d_off, bh->h_off, sizeof(int) * 6 * M, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(bh->d_off, bh->h_off, sizeof(int) * kOffArrays * M, hipMemcpyHostToDevice, s));
   Batch& b = bh->b;
   b.fr = bh->d_fr;
   int* d = bh->d_off;
@@ -459,6 +461,8 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   b.lab_blocks = o_lab[n];
   b.sm_blocks = o_sm[n];
   for (int sh = 0; sh < 3; sh++) b.tile_blocks[sh] = o_t[sh][n];
+  b.off.st = d + 6 * M;
+  b.st_blocks = o_st[n];
   b.work = bh->d_work;
   b.work_n = bh->d_work_n;
   b.work_cap = bh->work_cap;

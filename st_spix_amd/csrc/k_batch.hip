@@ -75,91 +75,125 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 
 // which: 0 = live labels into st (limit nbuf); 1 = split halves into sst
 // (limit nsm, ninvalid from the shifted map); 2 = live labels into sst.
+//
+// A workgroup covers a kStatTile x kStatTile tile: wave w its rows 8w..8w+7,
+// lane l the four pixels at column 4*(l&7) of row l>>3.  For each distinct
+// label of a wave (usually one to three) the lanes sum their pixels of that
+// label, DPP folds the sums into lane 63, and lane 63 issues one set of
+// atomics.  Position moments are summed relative to the tile origin in int32
+// and widened exactly: sum x = n*x0 + sum dx, sum x^2 = n*x0^2 + 2*x0*sum dx
+// + sum dx^2, sum xy = n*x0*y0 + x0*sum dy + y0*sum dx + sum dx*dy.
 template <bool BBOX>
-__global__ void kb_stats_accum(Batch b, int cur, int which) {
-  PIX_PROLOGUE
+__global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int which) {
+  const int f = frame_of(b.off.st, b.nf, blockIdx.x);
+  const FrameDesc& F = b.fr[f];
+  const int W = F.W, H = F.H;
+  const int tile = blockIdx.x - b.off.st[f];
+  const int tiles_x = (W + kStatTile - 1) / kStatTile;
+  const int tx0 = (tile % tiles_x) * kStatTile, ty0 = (tile / tiles_x) * kStatTile;
   const int lane = threadIdx.x & 63;
+  const int lx = 4 * (lane & 7), ly = 8 * (threadIdx.x >> 6) + (lane >> 3);
+  const int y = ty0 + ly;
   const int* labs = which == 1 ? F.halves : F.seg[cur];
   LabelStats st = which == 0 ? F.st : F.sst;
   const int nlab = which == 0 ? F.nbuf : F.nsm;
   const int* shifted = (which == 1 && F.prop_mode) ? F.shifted : nullptr;
-  const int W = F.W;
-  int k = -1;
-  int64_t v[11];
+  int k[4];
+  float col[4][3];
+  int inv[4];
+  unsigned pend = 0;
 #pragma unroll
-  for (int j = 0; j < 11; j++) v[j] = 0;
-  int cnt = 0, ninv = 0;
-  int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
-  if (t < F.npix) {
-    k = labs[t];
-    if (k >= nlab) k = -1;
-    if (k >= 0) {
-      const float* p = F.img + 3 * t;
-      float r = p[0], g = p[1], bl = p[2];
-      int64_t x = t % W, y = t / W;
-      v[0] = fx_color(r);
-      v[1] = fx_color(g);
-      v[2] = fx_color(bl);
-      v[3] = fx_square(r);
-      v[4] = fx_square(g);
-      v[5] = fx_square(bl);
-      v[6] = x;
-      v[7] = y;
-      v[8] = x * x;
-      v[9] = x * y;
-      v[10] = y * y;
-      cnt = 1;
-      if (shifted) ninv = shifted[t] < 0;
-      if (BBOX) {
-        bx0 = bx1 = (int)x;
-        by0 = by1 = (int)y;
+  for (int j = 0; j < 4; j++) {
+    const int x = tx0 + lx + j;
+    k[j] = -1;
+    inv[j] = 0;
+    col[j][0] = col[j][1] = col[j][2] = 0.f;
+    if (x < W && y < H) {
+      const int t = y * W + x;
+      const int v = labs[t];
+      if (v >= 0 && v < nlab) {
+        k[j] = v;
+        pend |= 1u << j;
+        const float* p = F.img + 3 * t;
+        col[j][0] = p[0];
+        col[j][1] = p[1];
+        col[j][2] = p[2];
+        if (shifted) inv[j] = shifted[t] < 0;
       }
     }
   }
-  int run_end;
-  const bool head = wave_run(k, run_end);
+  while (true) {
+    const uint64_t act = __ballot(pend != 0);
+    if (act == 0) break;
+    const int leader = __builtin_ctzll(act);
+    const int first = (pend & 1u) ? k[0] : (pend & 2u) ? k[1] : (pend & 4u) ? k[2] : k[3];
+    const int u = __builtin_amdgcn_readlane(first, leader);
+    int cnt = 0, sdx = 0, sdy = 0, sxx = 0, sxy = 0, syy = 0, ninv = 0;
+    int64_t S[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
+    int mnx = 0x7fffffff, mxx = -1;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int c2 = __shfl_down(cnt, d);
-    int n2 = __shfl_down(ninv, d);
-    int64_t w2[11];
+    for (int j = 0; j < 4; j++) {
+      if (((pend >> j) & 1u) && k[j] == u) {
+        pend &= ~(1u << j);
+        const int dx = lx + j;
+        cnt += 1;
+        sdx += dx;
+        sdy += ly;
+        sxx += dx * dx;
+        sxy += dx * ly;
+        syy += ly * ly;
+        ninv += inv[j];
 #pragma unroll
-    for (int j = 0; j < 11; j++) w2[j] = __shfl_down(v[j], d);
-    int ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0;
-    if (BBOX) {
-      ax0 = __shfl_down(bx0, d);
-      ax1 = __shfl_down(bx1, d);
-      ay0 = __shfl_down(by0, d);
-      ay1 = __shfl_down(by1, d);
-    }
-    if (lane + d < run_end) {
-      cnt += c2;
-      ninv += n2;
-#pragma unroll
-      for (int j = 0; j < 11; j++) v[j] += w2[j];
-      if (BBOX) {
-        bx0 = min(bx0, ax0);
-        bx1 = max(bx1, ax1);
-        by0 = min(by0, ay0);
-        by1 = max(by1, ay1);
+        for (int c = 0; c < 3; c++) {
+          S[c] += fx_color(col[j][c]);
+          Q[c] += fx_square(col[j][c]);
+        }
+        if (BBOX) {
+          mnx = min(mnx, tx0 + dx);
+          mxx = max(mxx, tx0 + dx);
+        }
       }
     }
-  }
-  if (head && k >= 0) {
-    atomicAdd(st.count + k, cnt);
-    if (ninv) atomicAdd(st.ninvalid + k, ninv);
+    int mny = cnt ? y : 0x7fffffff, mxy = cnt ? y : -1;
+    cnt = wave_sum_to63(cnt);
+    sdx = wave_sum_to63(sdx);
+    sdy = wave_sum_to63(sdy);
+    sxx = wave_sum_to63(sxx);
+    sxy = wave_sum_to63(sxy);
+    syy = wave_sum_to63(syy);
+    if (shifted) ninv = wave_sum_to63(ninv);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      atomic_add_i64(st.S + 3 * k + c, v[c]);
-      atomic_add_i64(st.Q + 3 * k + c, v[3 + c]);
+      S[c] = wave_sum64_to63(S[c]);
+      Q[c] = wave_sum64_to63(Q[c]);
     }
-#pragma unroll
-    for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * k + c, v[6 + c]);
     if (BBOX) {
-      atomicMin(st.bbox + 4 * k, bx0);
-      atomicMax(st.bbox + 4 * k + 1, bx1);
-      atomicMin(st.bbox + 4 * k + 2, by0);
-      atomicMax(st.bbox + 4 * k + 3, by1);
+      mnx = wave_min_to63(mnx);
+      mxx = wave_max_to63(mxx);
+      mny = wave_min_to63(mny);
+      mxy = wave_max_to63(mxy);
+    }
+    if (lane == 63) {
+      const int64_t n = cnt, X0 = tx0, Y0 = ty0;
+      atomicAdd(st.count + u, cnt);
+      if (ninv) atomicAdd(st.ninvalid + u, ninv);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        atomic_add_i64(st.S + 3 * u + c, S[c]);
+        atomic_add_i64(st.Q + 3 * u + c, Q[c]);
+      }
+      int64_t* X = st.X + 5 * u;
+      atomic_add_i64(X + 0, n * X0 + sdx);
+      atomic_add_i64(X + 1, n * Y0 + sdy);
+      atomic_add_i64(X + 2, n * X0 * X0 + 2 * X0 * sdx + sxx);
+      atomic_add_i64(X + 3, n * X0 * Y0 + X0 * sdy + Y0 * sdx + sxy);
+      atomic_add_i64(X + 4, n * Y0 * Y0 + 2 * Y0 * sdy + syy);
+      if (BBOX) {
+        atomicMin(st.bbox + 4 * u, mnx);
+        atomicMax(st.bbox + 4 * u + 1, mxx);
+        atomicMin(st.bbox + 4 * u + 2, mny);
+        atomicMax(st.bbox + 4 * u + 3, mxy);
+      }
     }
   }
 }
@@ -285,15 +319,15 @@ __global__ void kb_store_sample(Batch b) {
 void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, bbox ? 1 : 0);
   if (bbox)
-    kb_stats_accum<true><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+    kb_stats_accum<true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   else
-    kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+    kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
 void b_param_summary(const Batch& b, int cur, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
-  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 0);
+  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 1);
 }
 
@@ -970,7 +1004,7 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
   kb_bfs_block<<<256, 256, 0, s>>>(b, cur, direction);
-  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 1);
+  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
   kb_split_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
@@ -1121,7 +1155,7 @@ void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
-  kb_stats_accum<false><<<b.pix_blocks, kBlock, 0, s>>>(b, cur, 2);
+  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 2);
   kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_marg<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
