@@ -367,12 +367,24 @@ __device__ __forceinline__ int nbits(const int nb[8], int label) {
   return num;
 }
 
+// One workgroup updates a TW x TH tile for S sweeps from an LDS copy with a
+// halo of R = 4S.  Sweep s recomputes the border flags over the tile plus
+// 4(S-1-s)+3 cells, and phase p of it updates the cells with
+// (x%2, y%2) = (p>>1, p&1) within 4(S-1-s)+3-p cells of the tile: every
+// cell of the written tile then sees exactly the labels a full-frame sweep
+// would give it.  Border cells of each phase are compacted into LDS lists
+// while the flags are computed, so a phase is one dense pass (about one cell
+// per thread) with all its gathers in flight at once.  Cells of one phase are
+// never 8-neighbours, so the order inside a list does not matter.
 template <int TW, int TH, int S, int NT, int SHAPE>
 __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) {
   constexpr int R = 4 * S;
   constexpr int LW = TW + 2 * R, LH = TH + 2 * R;
+  constexpr int RB0 = 4 * S - 1;  // widest border ring (sweep 0)
+  constexpr int LCAP = ((TW + 2 * RB0 + 1) / 2) * ((TH + 2 * RB0 + 1) / 2);
   __shared__ int lab[LH * LW];
-  __shared__ uint8_t bord[LH * LW];
+  __shared__ unsigned short list[4][LCAP];
+  __shared__ int cnt[4];
   __shared__ uint32_t lutb[8];
   const int f = frame_of(b.off.tile[SHAPE], b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
@@ -393,43 +405,41 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     int gx = x0 + lx, gy = y0 + ly;
     lab[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? seg_in[gy * W + gx] : -1;
   }
-  __syncthreads();
   for (int s = 0; s < S; s++) {
+    if (tid < 4) cnt[tid] = 0;
+    __syncthreads();
     {
       const int rb = 4 * (S - 1 - s) + 3;
       const int bx0 = R - rb, by0 = R - rb, bw = TW + 2 * rb, bh = TH + 2 * rb;
       for (int i = tid; i < bw * bh; i += NT) {
-        int lx = bx0 + i % bw, ly = by0 + i / bw;
-        int gx = x0 + lx, gy = y0 + ly;
-        int li = ly * LW + lx;
-        uint8_t bb = 0;
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-          if (gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1) {
-            bb = 1;
-          } else {
-            int C = lab[li];
-            bb = (C != lab[li - LW]) || (C != lab[li + LW]) || (C != lab[li + 1]) ||
-                 (C != lab[li - 1]);
-          }
+        const int lx = bx0 + i % bw, ly = by0 + i / bw;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+        const int li = ly * LW + lx;
+        bool bb;
+        if (gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1) {
+          bb = true;
+        } else {
+          const int C = lab[li];
+          bb = (C != lab[li - LW]) || (C != lab[li + LW]) || (C != lab[li + 1]) ||
+               (C != lab[li - 1]);
         }
-        bord[li] = bb;
+        if (!bb) continue;
+        const int p = ((gx & 1) << 1) | (gy & 1);
+        const int rp = 4 * (S - 1 - s) + (3 - p);
+        if (lx < R - rp || lx >= R + TW + rp || ly < R - rp || ly >= R + TH + rp) continue;
+        const int pos = atomicAdd(&cnt[p], 1);
+        list[p][pos] = (unsigned short)li;
       }
       __syncthreads();
     }
 #pragma unroll 1
     for (int p = 0; p < 4; p++) {
-      const int xm = p >> 1, ym = p & 1;
-      const int rp = 4 * (S - 1 - s) + (3 - p);
-      const int rx0 = R - rp, ry0 = R - rp, rw = TW + 2 * rp, rh = TH + 2 * rp;
-      const int px0 = rx0 + (((xm - (x0 + rx0)) % 2) + 2) % 2;
-      const int py0 = ry0 + (((ym - (y0 + ry0)) % 2) + 2) % 2;
-      const int nx = (rx0 + rw - px0 + 1) / 2, ny = (ry0 + rh - py0 + 1) / 2;
-      for (int i = tid; i < nx * ny; i += NT) {
-        const int lx = px0 + 2 * (i % nx), ly = py0 + 2 * (i / nx);
+      const int n = cnt[p];
+      for (int i = tid; i < n; i += NT) {
+        const int li = list[p][i];
+        const int lx = li % LW, ly = li / LW;
         const int gx = x0 + lx, gy = y0 + ly;
-        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
-        const int li = ly * LW + lx;
-        if (!bord[li]) continue;
         const int C = lab[li];
         const int N = lab[li - LW], Sd = lab[li + LW], Wl = lab[li - 1], E = lab[li + 1];
         const int nb[8] = {lab[li - LW - 1], N, lab[li - LW + 1], Wl, E, lab[li + LW - 1], Sd,
