@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay the refinement as a hipGraph")
-    ap.add_argument("--cpu-sample-iters", type=int, default=100)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (counter calibration)")
     return ap.parse_args()
 
 
@@ -161,28 +162,54 @@ def run_served(args, streams, torch, core):
 
 
 def cpu_baseline(args):
-    """The oracle (single thread) on a bounded sample: one propagated
-    360x640 frame with the same iteration count, scaled by pixel count to
-    the 16-stream mix."""
-    import numpy as np
+    """The oracle on the host cores: one full tick of the stream mix (every
+    stream's propagated frame, same iteration count), one stream per thread
+    (the oracle has no shared mutable state; ctypes releases the GIL).  Frame
+    0 of each stream is an untimed 20-iteration BASS."""
+    import concurrent.futures as cf
 
     import oracle
     from st_spix_amd import synth
-    H, W = RESOLUTIONS[0]
-    vid, flows = synth.voronoi_video(2, H, W, nregions=120, vmax=4.0, seed=0)
-    cfg = oracle.make_cfg(niters=20, sp_size=args.sp_size, potts=0.5,
-                          prop_niters=args.cpu_sample_iters)
-    seg, p = oracle.bass(vid[0], cfg)
-    t0 = time.perf_counter()
-    oracle.prop(vid[1], flows[0], seg, p, cfg)
-    dt = time.perf_counter() - t0
-    px = sum(h * w for h, w in RESOLUTIONS) * args.streams_per_res
-    per_tick = dt * px / (H * W) * (args.iters / args.cpu_sample_iters)
-    nframes = len(RESOLUTIONS) * args.streams_per_res
-    return {"value": nframes / per_tick, "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C restatement), 1 thread: one {H}x{W} propagated frame, "
-                      f"{args.cpu_sample_iters} iterations, {dt:.2f} s; scaled by pixel "
-                      f"count to the {nframes}-stream mix"}
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    jobs = []
+    for ri, (H, W) in enumerate(RESOLUTIONS):
+        for j in range(args.streams_per_res):
+            vid, flows = synth.voronoi_video(2, H, W, nregions=120, vmax=4.0, seed=1000 * ri + j)
+            jobs.append((vid, flows))
+    cfg0 = oracle.make_cfg(niters=20, sp_size=args.sp_size, potts=0.5)
+    cfg = oracle.make_cfg(niters=20, sp_size=args.sp_size, potts=0.5, prop_niters=args.iters)
+
+    def first(job):
+        return oracle.bass(job[0][0], cfg0)
+
+    def tick(i):
+        vid, flows = jobs[i]
+        seg, p = states[i]
+        return oracle.prop(vid[1], flows[0], seg, p, cfg)
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        states = list(ex.map(first, jobs))
+        t0 = time.perf_counter()
+        list(ex.map(tick, range(len(jobs))))
+        dt = time.perf_counter() - t0
+    return {"value": len(jobs) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference), {threads} threads, one stream "
+                      f"per thread: one tick of all {len(jobs)} streams (a propagated frame each, "
+                      f"{args.iters} iterations) in {dt:.2f} s"}
+
+
+def sweep_traffic():
+    """HBM bytes per launch of the fused sweep over the same 16 frames, as
+    measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel)
+    and recorded in profiles/kb_sweep_traffic.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "kb_sweep_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -194,6 +221,14 @@ def main():
     import torch
 
     from st_spix_amd import core, synth
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
+        del src, dst
     streams = make_streams(args, torch, synth)
     log(f"{len(streams)} synthetic streams resident in HBM")
     served = run_served(args, streams, torch, core)
@@ -222,7 +257,7 @@ def main():
         "mpix_per_s": round(mpx, 2),
         "roofline": {"bound": "hbm", "kernel": "kb_sweep<64,32,4> (4 fused sweeps, all frames)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": sweep_traffic(),
                      "launches": served["sweep_launches"],
                      "avg_launch_ms": round(served["sweep_ms"] / max(1, served["sweep_launches"]), 4)},
     }
