@@ -26,6 +26,7 @@ RESOLUTIONS = [(360, 640), (480, 854), (720, 1280), (1080, 1920)]
 METRIC = "frames/s + p99 frame latency, 16 mixed-res streams, served vs plain loop"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PX_SWEEP = 20  # labels r 4 + img r 12 + labels w 4 per sweep
+SWEEPS_PER_ITER = 4      # niters_seg
 
 
 def parse():
@@ -36,7 +37,8 @@ def parse():
     ap.add_argument("--iters", type=int, default=100, help="refinement iterations per frame")
     ap.add_argument("--streams-per-res", type=int, default=4)
     ap.add_argument("--sp-size", type=int, default=25)
-    ap.add_argument("--sweep-shape", type=int, default=2)
+    ap.add_argument("--sweep-shape", type=int, default=4,
+                    help="fused sweep launch shape (index into core.SWEEP_SHAPES)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay the refinement as a hipGraph")
@@ -109,7 +111,7 @@ def run_plain(args, streams, torch, core):
         ms, n = ctx.kernel_time(0)
         sweep_ms += ms
         launches += n
-        sweep_bytes += n * BYTES_PER_PX_SWEEP * 4 * s["H"] * s["W"]  # 4 sweeps per launch
+        sweep_bytes += n * BYTES_PER_PX_SWEEP * SWEEPS_PER_ITER * s["H"] * s["W"]
         ctx.enable_timing(False)
     return {"elapsed": elapsed, "lat": lat, "sweep_ms": sweep_ms, "sweep_bytes": sweep_bytes,
             "sweep_launches": launches}
@@ -154,11 +156,14 @@ def run_served(args, streams, torch, core):
     while srv.pending():
         srv.step()
     torch.cuda.synchronize()
-    ms, n = lead.kernel_time(0)
+    ms, groups = lead.kernel_time(0)  # one event pair per 4-sweep group
     px = sum(s["H"] * s["W"] for s in streams)
+    per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
+    launches = groups * (SWEEPS_PER_ITER // per_launch)
     srv.close()
     return {"elapsed": elapsed, "lat": lat, "sweep_ms": ms,
-            "sweep_bytes": n * BYTES_PER_PX_SWEEP * 4 * px, "sweep_launches": n}
+            "sweep_bytes": launches * BYTES_PER_PX_SWEEP * per_launch * px,
+            "sweep_launches": launches}
 
 
 def cpu_baseline(args):
@@ -198,18 +203,23 @@ def cpu_baseline(args):
                       f"{args.iters} iterations) in {dt:.2f} s"}
 
 
-def sweep_traffic():
+def sweep_traffic(kernel):
     """HBM bytes per launch of the fused sweep over the same 16 frames, as
     measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel)
-    and recorded in profiles/kb_sweep_traffic.md; None if absent."""
+    and recorded in profiles/kb_sweep_traffic.md; None if absent or if it
+    was measured for another launch shape."""
     path = os.path.join(ROOT, "profiles", "kb_sweep_traffic.md")
+    fields = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+            if ":" in line and not line.startswith(("#", "|", " ", "`")):
+                key, val = line.split(":", 1)
+                fields[key.strip()] = val.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if fields.get("kernel") != kernel or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
 
 
 def main():
@@ -240,6 +250,7 @@ def main():
     fps = nframes / served["elapsed"]
     mpx = sum(s["H"] * s["W"] for s in streams) * args.steps / served["elapsed"] / 1e6
     achieved = served["sweep_bytes"] / (served["sweep_ms"] / 1e3) / 1e9 if served["sweep_ms"] else 0
+    kernel = "kb_sweep<%d,%d,%d>" % tuple(core.SWEEP_SHAPES[args.sweep_shape])
     line = {
         "metric": METRIC, "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -255,9 +266,10 @@ def main():
         "p50_ms": round(pct(served["lat"], 50) * 1e3, 3),
         "p99_ms": round(pct(served["lat"], 99) * 1e3, 3),
         "mpix_per_s": round(mpx, 2),
-        "roofline": {"bound": "hbm", "kernel": "kb_sweep<64,32,4> (4 fused sweeps, all frames)",
+        "roofline": {"bound": "hbm",
+                     "kernel": kernel + " (fused sweeps over all frames of the step)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": sweep_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": sweep_traffic(kernel),
                      "launches": served["sweep_launches"],
                      "avg_launch_ms": round(served["sweep_ms"] / max(1, served["sweep_launches"]), 4)},
     }

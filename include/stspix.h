@@ -80,9 +80,10 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
 void stsp_ctx_destroy(stsp_ctx* ctx);
 int stsp_ctx_set_config(stsp_ctx* ctx, const stsp_config* cfg);
 
-/* Launch shape of the fused boundary sweep: 0 = one sweep per launch
- * (32x16 tiles), 1 = two (32x32), 2 = four (64x32).  Results are identical
- * for every shape; only speed differs (chosen by the autotuner). */
+/* Launch shape of the fused boundary sweep (tile, sweeps per launch):
+ * 0 = 32x16 x1, 1 = 32x32 x2, 2 = 64x32 x4, 3 = 64x64 x2, 4 = 64x32 x2,
+ * 5 = 128x64 x2.  Results are identical for every shape; only speed
+ * differs.  Returns STSP_ERR_ARG outside 0..5. */
 int stsp_ctx_set_tuning(stsp_ctx* ctx, int sweep_shape);
 
 /* Capacity an output parameter set needs: BASS 30*K0 (bass.cu:245),

@@ -83,11 +83,17 @@ class Context:
 
 
 _CTX: Dict[tuple, Context] = {}
-_SWEEP_SHAPE = {"value": 2}
+
+# launch shapes of the fused sweep (tile width, height, sweeps per launch),
+# index = stsp_ctx_set_tuning's sweep_shape; the library default is 4
+SWEEP_SHAPES = [(32, 16, 1), (32, 32, 2), (64, 32, 4), (64, 64, 2), (64, 32, 2), (128, 64, 2)]
+DEFAULT_SWEEP_SHAPE = 4
+_SWEEP_SHAPE = {"value": DEFAULT_SWEEP_SHAPE}
 
 
 def set_sweep_shape(shape: int) -> None:
-    """Launch shape of the fused boundary sweep for all contexts (0, 1, 2)."""
+    """Launch shape of the fused boundary sweep for all contexts (0..5; see
+    stsp_ctx_set_tuning in include/stspix.h)."""
     _SWEEP_SHAPE["value"] = int(shape)
     for ctx in _CTX.values():
         ctx.set_tuning(int(shape))

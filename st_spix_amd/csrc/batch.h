@@ -45,23 +45,33 @@ struct FrameDesc {
   NigConsts nc;
 };
 
+// Launch shapes of the fused sweep: tile width, height, sweeps per launch,
+// threads.  A remainder of fewer sweeps than a shape fuses runs shape 0.
+struct SweepShape {
+  int tw, th, s, nt;
+};
+constexpr int kSweepShapes = 6;
+constexpr SweepShape kSweepShape[kSweepShapes] = {
+    {32, 16, 1, 64}, {32, 32, 2, 256}, {64, 32, 4, 256},
+    {64, 64, 2, 256}, {64, 32, 2, 256}, {128, 64, 2, 512}};
+
 // prefix offsets of one batch (device), each nf+1 long
 struct BatchOff {
   int* pix;      // 256-pixel blocks
   int* lab;      // 256-label blocks over cap
   int* sm;       // 256-slot blocks over 2*cap+2
-  int* tile[3];  // sweep tiles for each launch shape
+  int* tile[kSweepShapes];  // sweep tiles for each launch shape
   int* st;       // statistics tiles (kStatTile x kStatTile pixels)
 };
 
-constexpr int kOffArrays = 7;  // pix, lab, sm, tile[3], st
+constexpr int kOffArrays = 4 + kSweepShapes;  // pix, lab, sm, tile[], st
 constexpr int kStatTile = 32;
 
 struct Batch {
   FrameDesc* fr;  // device table
   BatchOff off;
   int nf;
-  int pix_blocks, lab_blocks, sm_blocks, tile_blocks[3], st_blocks;
+  int pix_blocks, lab_blocks, sm_blocks, tile_blocks[kSweepShapes], st_blocks;
   int* work;      // two work lists (frame << 24 | label) of work_cap entries each
   int* work_n;    // [0] length, [1] cursor of list 0; [2], [3] of list 1
   int work_cap;
@@ -69,8 +79,8 @@ struct Batch {
 
 // host-side block counts for one frame
 inline int tiles_for_shape(int shape, int H, int W) {
-  static const int tw[3] = {32, 32, 64}, th[3] = {16, 32, 32};
-  return ((W + tw[shape] - 1) / tw[shape]) * ((H + th[shape] - 1) / th[shape]);
+  const SweepShape& q = kSweepShape[shape];
+  return ((W + q.tw - 1) / q.tw) * ((H + q.th - 1) / q.th);
 }
 
 inline int stat_tiles(int H, int W) {

@@ -27,7 +27,11 @@ struct alignas(16) SweepRec {
   float logdet;  // logdet_sigma_shape narrowed to float
   float isig[3]; // inverse shape covariance narrowed to float (xx, xy, yy)
   int count;
-  double mx, my; // mean position (double: the update truncates x - mu to int)
+  // ceil and floor of the mean position.  The mean is X/n for integer sums
+  // (or 0), which is either an integer or at least 1/n away from one, so
+  // trunc((double)x - mean) == x - ceil(mean) when x >= ceil(mean) and
+  // x - floor(mean) otherwise, exactly, for integer x.
+  int cx, fx, cy, fy;
 };
 
 // Per-label sufficient statistics.  S = colour sums, Q = colour square sums

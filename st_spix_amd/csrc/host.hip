@@ -115,7 +115,7 @@ struct stsp_ctx {
   stsp_config cfg;
   int H, W, npix;
   int cap = 0, nprev = 0;
-  int sweep_shape = 2;  // 0: 1 sweep/launch, 1: 2, 2: 4
+  int sweep_shape = 4;  // index into kSweepShape (64x32 tiles, 2 sweeps per launch)
   int timing = 0;
   KTimer timers[STSP_KCLASS_COUNT];
   SimpleLut lut;
@@ -424,17 +424,20 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   int* o_pix = off;
   int* o_lab = off + M;
   int* o_sm = off + 2 * M;
-  int* o_t[3] = {off + 3 * M, off + 4 * M, off + 5 * M};
-  int* o_st = off + 6 * M;
+  int* o_t[kSweepShapes];
+  for (int sh = 0; sh < kSweepShapes; sh++) o_t[sh] = off + (3 + sh) * M;
+  int* o_st = off + (3 + kSweepShapes) * M;
   int work = 0;
-  o_pix[0] = o_lab[0] = o_sm[0] = o_t[0][0] = o_t[1][0] = o_t[2][0] = o_st[0] = 0;
+  o_pix[0] = o_lab[0] = o_sm[0] = o_st[0] = 0;
+  for (int sh = 0; sh < kSweepShapes; sh++) o_t[sh][0] = 0;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = ctxs[i];
     bh->h_fr[i] = make_desc(c);
     o_pix[i + 1] = o_pix[i] + div_up(c->npix, kBlock);
     o_lab[i + 1] = o_lab[i] + div_up(c->cap, kBlock);
     o_sm[i + 1] = o_sm[i] + div_up(2 * c->cap + 2, kBlock);
-    for (int sh = 0; sh < 3; sh++) o_t[sh][i + 1] = o_t[sh][i] + tiles_for_shape(sh, c->H, c->W);
+    for (int sh = 0; sh < kSweepShapes; sh++)
+      o_t[sh][i + 1] = o_t[sh][i] + tiles_for_shape(sh, c->H, c->W);
     o_st[i + 1] = o_st[i] + stat_tiles(c->H, c->W);
     work += c->cap;
   }
@@ -455,13 +458,13 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   b.off.pix = d;
   b.off.lab = d + M;
   b.off.sm = d + 2 * M;
-  for (int sh = 0; sh < 3; sh++) b.off.tile[sh] = d + (3 + sh) * M;
+  for (int sh = 0; sh < kSweepShapes; sh++) b.off.tile[sh] = d + (3 + sh) * M;
   b.nf = n;
   b.pix_blocks = o_pix[n];
   b.lab_blocks = o_lab[n];
   b.sm_blocks = o_sm[n];
-  for (int sh = 0; sh < 3; sh++) b.tile_blocks[sh] = o_t[sh][n];
-  b.off.st = d + 6 * M;
+  for (int sh = 0; sh < kSweepShapes; sh++) b.tile_blocks[sh] = o_t[sh][n];
+  b.off.st = d + (3 + kSweepShapes) * M;
   b.st_blocks = o_st[n];
   b.work = bh->d_work;
   b.work_n = bh->d_work_n;
@@ -777,7 +780,7 @@ int stsp_ctx_set_config(stsp_ctx* c, const stsp_config* cfg) {
 }
 
 int stsp_ctx_set_tuning(stsp_ctx* c, int sweep_shape) {
-  if (!c || sweep_shape < 0 || sweep_shape > 2) return STSP_ERR_ARG;
+  if (!c || sweep_shape < 0 || sweep_shape >= kSweepShapes) return STSP_ERR_ARG;
   c->sweep_shape = sweep_shape;
   return STSP_OK;
 }

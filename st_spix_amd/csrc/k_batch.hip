@@ -291,8 +291,11 @@ __global__ void kb_param_update(Batch b, int summary_mode) {
   r.isig[1] = (float)P.sigma_shape[3 * k + 1];
   r.isig[2] = (float)P.sigma_shape[3 * k + 2];
   r.count = P.count[k];
-  r.mx = P.mu_shape[2 * k];
-  r.my = P.mu_shape[2 * k + 1];
+  const double mx = P.mu_shape[2 * k], my = P.mu_shape[2 * k + 1];
+  r.cx = (int)ceil(mx);
+  r.fx = (int)floor(mx);
+  r.cy = (int)ceil(my);
+  r.fy = (int)floor(my);
   F.rec[k] = r;
 }
 
@@ -346,8 +349,8 @@ __device__ __forceinline__ void joint_b(const float* imgC, int x, int y, const S
   const float x0 = imgC[0] - r.mu[0];
   const float x1 = imgC[1] - r.mu[1];
   const float x2 = imgC[2] - r.mu[2];
-  const int d0 = (int)((double)x - r.mx);
-  const int d1 = (int)((double)y - r.my);
+  const int d0 = x >= r.cx ? x - r.cx : x - r.fx;  // == (int)((double)x - mean x)
+  const int d1 = y >= r.cy ? y - r.cy : y - r.fy;
   res = res - (x0 * x0 + x1 * x1 + x2 * x2) / sigma2_app;
   res = res - (float)(d0 * d0) * r.isig[0] - (float)(d1 * d1) * r.isig[2] -
         (float)(2 * d0 * d1) * r.isig[1];
@@ -480,22 +483,26 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
 // runs niters_seg sweeps; returns the index of the buffer holding the result
 int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut& lut,
              hipStream_t s) {
+#define STSP_SWEEP(SH)                                                                   \
+  kb_sweep<kSweepShape[SH].tw, kSweepShape[SH].th, kSweepShape[SH].s, kSweepShape[SH].nt, \
+           SH><<<b.tile_blocks[SH], kSweepShape[SH].nt, 0, s>>>(b, cur, lut)
   int done = 0;
   while (done < niters_seg) {
-    int left = niters_seg - done;
-    if (shape == 2 && left >= 4) {
-      kb_sweep<64, 32, 4, 256, 2><<<b.tile_blocks[2], 256, 0, s>>>(b, cur, lut);
-      done += 4;
-    } else if (shape >= 1 && left >= 2) {
-      kb_sweep<32, 32, 2, 256, 1><<<b.tile_blocks[1], 256, 0, s>>>(b, cur, lut);
-      done += 2;
-    } else {
-      kb_sweep<32, 16, 1, 64, 0><<<b.tile_blocks[0], 64, 0, s>>>(b, cur, lut);
-      done += 1;
+    int sh = shape;
+    if (kSweepShape[sh].s > niters_seg - done) sh = 0;
+    switch (sh) {
+      case 1: STSP_SWEEP(1); break;
+      case 2: STSP_SWEEP(2); break;
+      case 3: STSP_SWEEP(3); break;
+      case 4: STSP_SWEEP(4); break;
+      case 5: STSP_SWEEP(5); break;
+      default: STSP_SWEEP(0); break;
     }
+    done += kSweepShape[sh].s;
     cur ^= 1;
   }
   return cur;
+#undef STSP_SWEEP
 }
 
 // ---------------------------------------------------------------------------

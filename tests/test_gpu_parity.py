@@ -92,7 +92,7 @@ def test_split_bfs_box_sizes(orc, H, W, sp, niters, nreg, seed):
     assert_params(params, p_r, f"{H}x{W} sp{sp} params")
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_sweep_shapes_bit_identical(orc, shape):
     """Every launch shape of the fused sweep (1, 2 or 4 sweeps per launch,
     different tiles) gives the oracle's labels; odd sizes exercise ragged
@@ -104,7 +104,7 @@ def test_sweep_shapes_bit_identical(orc, shape):
     try:
         spix, params = gpu_bass(img, **kw)
     finally:
-        core.set_sweep_shape(2)
+        core.set_sweep_shape(core.DEFAULT_SWEEP_SHAPE)
     seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
     assert_labels(spix, seg_r, f"shape{shape}")
     assert_params(params, p_r, f"shape{shape}")
