@@ -15,6 +15,7 @@
 #define STSPIX_H
 
 #include <hip/hip_runtime_api.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,6 +47,12 @@ typedef struct {
   float merge_alpha;  /* pyapi.cu:65, 0 */
   float thresh_relabel; /* pyapi.cu:68, 1e-5 */
   float thresh_new;   /* pyapi.cu:69, 1e-2 */
+  int target_nspix;   /* >0: steer the split/merge alphas every second iteration
+                         until the label count is within 5% of this (and at
+                         least niters ran), at most 5000 iterations
+                         (bass.cu:65-105, prop.cu:356-401; pyapi passes 0).
+                         Single-frame calls only: the batch calls return
+                         STSP_ERR_ARG for such contexts. */
 } stsp_config;
 
 /* Parameter arrays of one frame: the field set of the reference
@@ -120,11 +127,33 @@ int stsp_prop_batch(stsp_batch* batch, stsp_ctx** ctxs, int n, const float** img
                     const int* nspix_prev, int** spix_out, stsp_params* outs, int* nspix_out,
                     int* niters_state, hipStream_t stream);
 
+/* Batched BASS over n images of any sizes (one context each): hexagonal
+ * initialisation per image, then the refinement of all images in shared
+ * launches (images with the same schedule together), then compaction and
+ * export per image.  Per image the same result as stsp_bass.  Replaces a
+ * loop of bass() calls over a set of images (bass.cu:178-330 per image;
+ * indepent_bass, prop/__init__.py:38-85). */
+int stsp_bass_batch(stsp_batch* batch, stsp_ctx** ctxs, int n, const float** imgs,
+                    int** spix_out, stsp_params* outs, int* nspix_out, hipStream_t stream);
+
 /* Border overlay of get_marked_video (pyapi.cu:221-303,
  * find_border_pixels_end seg_utils.cu:218-257): marked = clamp(vid,0,1)
  * except label-border pixels, which take color[3]. nframes images. */
 int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,
                       int height, int width, float* marked, hipStream_t stream);
+
+/* Colour preprocessing of the command-line path (demo.cu:248-250).
+ * stsp_rescale_image: out = img_u8 / scale for 3*npix values per image
+ * (rescale_image, rgb2lab.cu:19-32, on a device buffer).
+ * stsp_rgb2lab: B,G,R floats in [0,1] -> Lab scaled to (L/-100, a/100,
+ * b/100) (rgb2lab / rgb_to_lab, rgb2lab.cu:35-117).
+ * stsp_lab2rgb: the inverse, writing R,G,B clamped to [0,1] (lab2rgb /
+ * lab_to_rgb, rgb2lab.cu:42-162).  Pixels with a NaN channel are left
+ * unwritten.  npix = pixels per image. */
+int stsp_rescale_image(const uint8_t* img_u8, float* out, int nimages, int npix, float scale,
+                       hipStream_t stream);
+int stsp_rgb2lab(const float* img_bgr, float* img_lab, int nimages, int npix, hipStream_t stream);
+int stsp_lab2rgb(const float* img_lab, float* img_rgb, int nimages, int npix, hipStream_t stream);
 
 /* Device-time accounting of the hot kernels, measured with hipEvent pairs
  * on the launch stream around each launch of the class (no counterpart in

@@ -41,7 +41,7 @@ class Cfg(ctypes.Structure):
                 ("sigma2_app", ctypes.c_float), ("potts", ctypes.c_float),
                 ("alpha_hastings", ctypes.c_float), ("split_alpha", ctypes.c_float),
                 ("merge_alpha", ctypes.c_float), ("thresh_relabel", ctypes.c_float),
-                ("thresh_new", ctypes.c_float)]
+                ("thresh_new", ctypes.c_float), ("target_nspix", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
@@ -72,7 +72,7 @@ def make_cfg(**kw) -> Cfg:
     d = dict(niters=500, niters_seg=4, sm_start=0, sp_size=25, sm_period=4,
              prop_niters=0, sigma2_app=8.1e-5, potts=10.0,
              alpha_hastings=float(np.log(0.5)), split_alpha=0.0, merge_alpha=0.0,
-             thresh_relabel=1e-5, thresh_new=1e-2)
+             thresh_relabel=1e-5, thresh_new=1e-2, target_nspix=0)
     d.update(kw)
     return Cfg(**d)
 

@@ -945,6 +945,49 @@ static int compactify(int* seg, int npix, osp* P, int nbuf, int prev_nspix, int 
   return prev_nspix + num_new;
 }
 
+/* Number of distinct labels in seg (extract_unique_ids(seg).size(),
+ * compact_spix.cu:196-230, as used at bass.cu:75 and prop.cu:373). */
+static int count_living(const int* seg, int npix, int nbuf) {
+  unsigned char* seen = (unsigned char*)calloc((size_t)nbuf, 1);
+  int n = 0;
+  for (int t = 0; t < npix; t++) {
+    int k = seg[t];
+    if (k >= 0 && k < nbuf && !seen[k]) {
+      seen[k] = 1;
+      n++;
+    }
+  }
+  free(seen);
+  return n;
+}
+
+/* Split/merge alpha control toward target_nspix, bass.cu:73-98 and
+ * prop.cu:371-400 (the prop variant also doubles thresh_new every 200
+ * iterations past 2000).  Returns 1 when the count is accepted. */
+static int control_nspix(int nliving, int target, int idx, int og_niters, int prop_mode,
+                         float* split_alpha, float* merge_alpha, float* thresh_new) {
+  if (nliving > 1.05 * target) {
+    if (*split_alpha > 0) {
+      *split_alpha = 0;
+      *merge_alpha = 0;
+    }
+    *split_alpha += -1;
+    *merge_alpha += 1.0;
+  } else if (nliving < 0.95 * target) {
+    if (*split_alpha < 0) {
+      *split_alpha = 0;
+      *merge_alpha = 0;
+    }
+    *split_alpha += 1;
+    *merge_alpha += -1;
+  } else {
+    *split_alpha = 0.;
+    *merge_alpha = 0.;
+  }
+  if (prop_mode && idx > 2000 && idx % 200 == 0) *thresh_new = (float)(2 * (*thresh_new + 1e-5));
+  return nliving < 1.05 * target && nliving > 0.95 * target && idx >= og_niters;
+}
+
 static int max_label(const int* seg, int npix) {
   int m = -1;
   for (int i = 0; i < npix; i++)
@@ -1109,18 +1152,27 @@ int orc_bass(const float* img, int height, int width, const orc_cfg* cfg, int* s
   double log_alpha = stsp_det_log((double)expf(cfg->alpha_hastings));
   int smc = 1; /* split/merge counter, bass.cu:56 */
   int max_spix = nspix - 1;
-  for (int idx = 0; idx < cfg->niters; idx++) {
+  /* bass.cu:60-67: merge_alpha is local (0) in BASS; 5000-iteration cap
+   * when the count is controlled */
+  float split_alpha = cfg->split_alpha, merge_alpha = 0.0f, thresh_new = cfg->thresh_new;
+  const int controlled = cfg->target_nspix > 0;
+  const int niters = controlled ? 5000 : cfg->niters;
+  for (int idx = 0; idx < niters; idx++) {
+    if (controlled && idx % 2 == 0 &&
+        control_nspix(count_living(seg, npix, nbuf), cfg->target_nspix, idx, cfg->niters, 0,
+                      &split_alpha, &merge_alpha, &thresh_new))
+      break;
     update_params(img, seg, npix, width, P, st, nbuf, sp_size);
     if (is_split_iter(idx, cfg)) {
       smc += 1; /* run_split_orig, split_merge_orig.cu:37-52 */
       int direction = smc % 2 + 1;
       max_spix = split_step(img, seg, NULL, P, st, nbuf, max_spix, direction, &nc,
-                            cfg->split_alpha, sp_size, 0, width, height, &w);
+                            split_alpha, sp_size, 0, width, height, &w);
       update_params(img, seg, npix, width, P, st, nbuf, sp_size);
     }
     if (is_merge_iter(idx, cfg)) {
       int direction = smc % 2; /* run_merge_orig, split_merge_orig.cu:67-75 */
-      merge_step(img, seg, border, P, st, nbuf, direction, &nc, log_alpha, 0.0f, 0, width,
+      merge_step(img, seg, border, P, st, nbuf, direction, &nc, log_alpha, merge_alpha, 0, width,
                  height, &w);
       update_params(img, seg, npix, width, P, st, nbuf, sp_size);
     }
@@ -1487,21 +1539,30 @@ int orc_prop(const float* img, const float* flow, int height, int width,
   nig_consts nc = make_nig(cfg->sigma2_app);
   double log_alpha = stsp_det_log((double)expf(cfg->alpha_hastings));
   int smc = 1, max_spix = nspix - 1;
+  float split_alpha = cfg->split_alpha, merge_alpha = cfg->merge_alpha,
+        thresh_new = cfg->thresh_new;
+  const int controlled = cfg->target_nspix > 0; /* prop.cu:356-360 */
+  const int og_niters = niters;
+  if (controlled) niters = 5000;
   for (int idx = 0; idx < niters; idx++) {
     update_params(img, seg, npix, width, P, st, nbuf, sp_size);
+    if (controlled && idx % 2 == 0 &&
+        control_nspix(count_living(seg, npix, nbuf), cfg->target_nspix, idx, og_niters, 1,
+                      &split_alpha, &merge_alpha, &thresh_new))
+      break;
     if (is_split_iter(idx, cfg)) {
       smc += 1; /* run_split_p, split_merge_prop.cu:31-59 */
       int direction = smc % 2 + 1;
       max_spix = split_step(img, seg, shifted, P, st, nbuf, max_spix, direction, &nc,
-                            cfg->split_alpha, sp_size, 1, width, height, &w);
+                            split_alpha, sp_size, 1, width, height, &w);
       update_params(img, seg, npix, width, P, st, nbuf, sp_size);
     }
     if (is_merge_iter(idx, cfg)) {
-      max_spix = relabel_step(seg, npix, P, nbuf, prev, cfg->thresh_relabel,
-                              cfg->thresh_new, nspix_prev, max_spix);
+      max_spix = relabel_step(seg, npix, P, nbuf, prev, cfg->thresh_relabel, thresh_new,
+                              nspix_prev, max_spix);
       smc += 1; /* run_merge_p, split_merge_prop.cu:62-85 */
       int direction = smc % 2;
-      merge_step(img, seg, border, P, st, nbuf, direction, &nc, log_alpha, cfg->merge_alpha,
+      merge_step(img, seg, border, P, st, nbuf, direction, &nc, log_alpha, merge_alpha,
                  1, width, height, &w);
       update_params(img, seg, npix, width, P, st, nbuf, sp_size);
     }

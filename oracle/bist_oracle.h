@@ -32,6 +32,7 @@ typedef struct {
   float merge_alpha;
   float thresh_relabel;
   float thresh_new;
+  int target_nspix;    /* >0: steer split/merge alphas to this count (bass.cu:65-105) */
 } orc_cfg;
 
 /* SoA parameter set, field-for-field the reference SuperpixelParams

@@ -30,7 +30,7 @@ class Config(ctypes.Structure):
                 ("sigma2_app", ctypes.c_float), ("potts", ctypes.c_float),
                 ("alpha_hastings", ctypes.c_float), ("split_alpha", ctypes.c_float),
                 ("merge_alpha", ctypes.c_float), ("thresh_relabel", ctypes.c_float),
-                ("thresh_new", ctypes.c_float)]
+                ("thresh_new", ctypes.c_float), ("target_nspix", ctypes.c_int)]
 
 
 PARAM_FIELDS = [
@@ -61,7 +61,8 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_set_config", "stsp_bass_capacity", "stsp_prop_capacity", "stsp_bass",
            "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
            "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
-           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_prop_batch"]
+           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_prop_batch", "stsp_bass_batch",
+           "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb"]
 
 
 def lib() -> ctypes.CDLL:
@@ -97,6 +98,11 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(vp), ctypes.POINTER(vp),
                                       ctypes.POINTER(ParamsStruct), pi, ctypes.POINTER(vp),
                                       ctypes.POINTER(ParamsStruct), pi, pi, vp]
+        L.stsp_bass_batch.argtypes = [vp, ctypes.POINTER(vp), i, ctypes.POINTER(vp),
+                                      ctypes.POINTER(vp), ctypes.POINTER(ParamsStruct), pi, vp]
+        L.stsp_rescale_image.argtypes = [vp, vp, i, i, ctypes.c_float, vp]
+        L.stsp_rgb2lab.argtypes = [vp, vp, i, i, vp]
+        L.stsp_lab2rgb.argtypes = [vp, vp, i, i, vp]
         L.stsp_status_string.argtypes = [i]
         L.stsp_status_string.restype = ctypes.c_char_p
         _lib = L

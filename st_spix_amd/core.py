@@ -29,12 +29,14 @@ def make_config(niters: int = 500, sp_size: int = 25, potts: float = 0.5,
                 sigma2_app: float = 8.1e-5, alpha: float = math.log(0.5),
                 split_alpha: float = 0.0, merge_alpha: float = 0.0, niters_seg: int = 4,
                 sm_start: int = 0, sm_period: int = 4, prop_niters: int = 0,
-                thresh_relabel: float = 1e-5, thresh_new: float = 1e-2) -> _lib.Config:
+                thresh_relabel: float = 1e-5, thresh_new: float = 1e-2,
+                target_nspix: int = 0) -> _lib.Config:
     return _lib.Config(niters=niters, niters_seg=niters_seg, sm_start=sm_start,
                        sp_size=sp_size, sm_period=sm_period, prop_niters=prop_niters,
                        sigma2_app=sigma2_app, potts=potts, alpha_hastings=alpha,
                        split_alpha=split_alpha, merge_alpha=merge_alpha,
-                       thresh_relabel=thresh_relabel, thresh_new=thresh_new)
+                       thresh_relabel=thresh_relabel, thresh_new=thresh_new,
+                       target_nspix=target_nspix)
 
 
 class Context:
@@ -237,6 +239,39 @@ class BatchRunner:
             self.close()
         except Exception:
             pass
+
+    def bass(self, items, stream=None):
+        """items: list of (ctx, img).  First frames (BASS) of several images or
+        streams in shared launches.  Returns a list of (spix, params)."""
+        n = len(items)
+        if n == 0:
+            return []
+        if n > self.max_frames:
+            raise RuntimeError(f"batch of {n} exceeds max_frames={self.max_frames}")
+        VP = ctypes.c_void_p
+        ctxs, imgs, souts = (VP * n)(), (VP * n)(), (VP * n)()
+        outs = (_lib.ParamsStruct * n)()
+        nout = (ctypes.c_int * n)()
+        keep = []
+        for i, (ctx, img) in enumerate(items):
+            _check_input(img, "img")
+            if img.dtype != torch.float32 or img.dim() != 3 or img.shape[2] != 3:
+                raise RuntimeError("img must be float32 (H, W, 3)")
+            if tuple(img.shape[:2]) != (ctx.H, ctx.W):
+                raise RuntimeError("img size does not match its context")
+            cap = _lib.lib().stsp_bass_capacity(ctx.cfg.sp_size, ctx.H, ctx.W)
+            out = alloc_params(cap, img.device)
+            spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
+            ctxs[i] = ctx.handle.value
+            imgs[i] = img.data_ptr()
+            souts[i] = spix.data_ptr()
+            outs[i] = params_struct(out, cap)
+            keep.append((spix, out))
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = _lib.lib().stsp_bass_batch(self.handle, ctxs, n, imgs, souts, outs, nout,
+                                        ctypes.c_void_p(s.cuda_stream))
+        _lib.check(rc, "stsp_bass_batch")
+        return [(keep[i][0], trim(keep[i][1], nout[i])) for i in range(n)]
 
     def prop(self, items, stream=None):
         """items: list of (ctx, img, flow, spix_prev, params_prev).  Returns a

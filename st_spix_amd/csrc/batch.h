@@ -96,5 +96,6 @@ int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut
 void b_split(const Batch& b, int cur, int direction, hipStream_t s);
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s);
 void b_relabel(const Batch& b, int cur, hipStream_t s);
+void b_count_living(const Batch& b, int slot, hipStream_t s);
 
 }  // namespace stsp

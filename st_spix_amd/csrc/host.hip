@@ -144,6 +144,9 @@ struct stsp_ctx {
   void* prev_mem = nullptr;
   // per-frame state between prologue and epilogue
   int nbuf = 0, nspix_prev = 0, prop_mode = 0, niters = 0;
+  // split/merge alphas and the new-label threshold of the current frame (the
+  // label-count control changes them between iterations)
+  float split_alpha = 0.f, merge_alpha = 0.f, thresh_new = 0.f;
   const float* img = nullptr;
   const float* prev_prior_mu_app = nullptr;
   stsp_batch self;  // batch of one for single-frame calls
@@ -405,10 +408,10 @@ static FrameDesc make_desc(stsp_ctx* c) {
   d.dead = c->dead;
   d.sigma2_app = c->cfg.sigma2_app;
   d.potts = c->cfg.potts;
-  d.split_alpha = c->cfg.split_alpha;
-  d.merge_alpha = c->cfg.merge_alpha;
+  d.split_alpha = c->split_alpha;
+  d.merge_alpha = c->merge_alpha;
   d.thresh_relabel = c->cfg.thresh_relabel;
-  d.thresh_new = c->cfg.thresh_new;
+  d.thresh_new = c->thresh_new;
   d.sp_size = c->cfg.sp_size;
   d.prop_mode = c->prop_mode;
   d.nc = make_nig(c->cfg);
@@ -481,9 +484,33 @@ static inline bool is_merge_iter(const stsp_config& cfg, int idx) {
   return idx > cfg.sm_start && idx % p == p / 2;
 }
 
-// The refinement loop of bass() (bass.cu:71-168) / prop_bass_v3
-// (prop.cu:364-457) for every frame of the batch; returns the live buffer
-// index.  `lead` supplies the settings shared by the batch and the timers.
+// One outer iteration of bass() (bass.cu:71-168) / prop_bass_v3
+// (prop.cu:364-457) for every frame of the batch; the update of the
+// parameters at its start is done by the caller.  `lead` supplies the
+// settings shared by the batch and the timers.
+static void iteration_tail(stsp_ctx* lead, const Batch& b, int idx, int prop_mode, int shape,
+                           int* cur, int* smc, hipStream_t s) {
+  const stsp_config& cfg = lead->cfg;
+  if (is_split_iter(cfg, idx)) {
+    *smc += 1;
+    b_split(b, *cur, *smc % 2 + 1, s);
+    b_update_params(b, *cur, false, s);
+  }
+  if (is_merge_iter(cfg, idx)) {
+    if (prop_mode) {
+      b_relabel(b, *cur, s);
+      *smc += 1;
+    }
+    b_merge(b, *cur, *smc % 2, s);
+    b_update_params(b, *cur, false, s);
+  }
+  tstart(lead, STSP_KCLASS_SWEEP, s);
+  *cur = b_sweeps(b, *cur, cfg.niters_seg, shape, lead->lut, s);
+  tstop(lead, STSP_KCLASS_SWEEP, s);
+}
+
+// The refinement loop for every frame of the batch; returns the live
+// buffer index.
 static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_mode,
                            int shape, hipStream_t s) {
   const stsp_config& cfg = lead->cfg;
@@ -493,31 +520,85 @@ static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_
     tstart(lead, STSP_KCLASS_STATS, s);
     b_update_params(b, cur, is_split_iter(cfg, idx), s);
     tstop(lead, STSP_KCLASS_STATS, s);
-    if (is_split_iter(cfg, idx)) {
-      smc += 1;
-      b_split(b, cur, smc % 2 + 1, s);
-      b_update_params(b, cur, false, s);
-    }
-    if (is_merge_iter(cfg, idx)) {
-      if (prop_mode) {
-        b_relabel(b, cur, s);
-        smc += 1;
-      }
-      b_merge(b, cur, smc % 2, s);
-      b_update_params(b, cur, false, s);
-    }
-    tstart(lead, STSP_KCLASS_SWEEP, s);
-    cur = b_sweeps(b, cur, cfg.niters_seg, shape, lead->lut, s);
-    tstop(lead, STSP_KCLASS_SWEEP, s);
+    iteration_tail(lead, b, idx, prop_mode, shape, &cur, &smc, s);
   }
   b_update_params(b, cur, false, s);
   b_store_sample(b, s);
   return cur;
 }
 
+// Split/merge alpha control toward target_nspix (bass.cu:73-98,
+// prop.cu:371-400; the prop variant also doubles thresh_new every 200
+// iterations past 2000).  Returns true when the count is accepted.
+static bool control_nspix(int nliving, int target, int idx, int og_niters, int prop_mode,
+                          float* split_alpha, float* merge_alpha, float* thresh_new) {
+  if (nliving > 1.05 * target) {
+    if (*split_alpha > 0) {
+      *split_alpha = 0;
+      *merge_alpha = 0;
+    }
+    *split_alpha += -1;
+    *merge_alpha += 1.0;
+  } else if (nliving < 0.95 * target) {
+    if (*split_alpha < 0) {
+      *split_alpha = 0;
+      *merge_alpha = 0;
+    }
+    *split_alpha += 1;
+    *merge_alpha += -1;
+  } else {
+    *split_alpha = 0.;
+    *merge_alpha = 0.;
+  }
+  if (prop_mode && idx > 2000 && idx % 200 == 0) *thresh_new = (float)(2 * (*thresh_new + 1e-5));
+  return nliving < 1.05 * target && nliving > 0.95 * target && idx >= og_niters;
+}
+
+// Refinement of one frame with the label-count control: every second
+// iteration the number of distinct labels (labels with pixels after the
+// parameter update) is read back and the alphas are adjusted; the loop ends
+// when the count is accepted or after 5000 iterations.  The reference counts
+// before the update in BASS and after it in prop; the update does not change
+// the labels and repeating it is idempotent, so both orders give the same
+// result.
+static int refine_controlled(stsp_batch* bh, stsp_ctx* c, hipStream_t s) {
+  stsp_ctx* one = c;
+  int rc = batch_build(bh, &one, 1, s);
+  if (rc) return rc;
+  const stsp_config& cfg = c->cfg;
+  const int og_niters = c->niters;
+  int cur = 0, smc = 1;
+  for (int idx = 0; idx < 5000; idx++) {
+    b_update_params(bh->b, cur, is_split_iter(cfg, idx), s);
+    if (idx % 2 == 0) {
+      k_set_int<<<1, 1, 0, s>>>(c->counter + 1, 0);
+      b_count_living(bh->b, 1, s);
+      int nliving = 0;
+      rc = read_int(c, c->counter + 1, s, &nliving);
+      if (rc) return rc;
+      if (control_nspix(nliving, cfg.target_nspix, idx, og_niters, c->prop_mode, &c->split_alpha,
+                        &c->merge_alpha, &c->thresh_new))
+        break;
+      rc = batch_build(bh, &one, 1, s);  // the stream is idle here: the table can be rewritten
+      if (rc) return rc;
+    }
+    iteration_tail(c, bh->b, idx, c->prop_mode, c->sweep_shape, &cur, &smc, s);
+  }
+  b_update_params(bh->b, cur, false, s);
+  b_store_sample(bh->b, s);
+  if (cur == 1) {
+    int* t = c->seg;
+    c->seg = c->seg2;
+    c->seg2 = t;
+  }
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
 // Runs the refinement of frames ctxs[0..n) (prologues done) as one batch.
 static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   stsp_ctx* lead = ctxs[0];
+  if (lead->cfg.target_nspix > 0) return n == 1 ? refine_controlled(bh, lead, s) : STSP_ERR_ARG;
   const int niters = lead->niters, prop_mode = lead->prop_mode, shape = lead->sweep_shape;
   for (int i = 1; i < n; i++) {
     const stsp_config& a = ctxs[i]->cfg;
@@ -530,7 +611,9 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   if (rc) return rc;
   int cur;
   bool timing = lead->timing != 0;
-  if (bh->use_graph && !timing) {
+  // The legacy default stream cannot be captured: there the launches run
+  // eagerly (same kernels, same results).
+  if (bh->use_graph && !timing && s != nullptr) {
     std::vector<long long> key = {n, niters, prop_mode, shape, lead->cfg.niters_seg,
                                   lead->cfg.sm_period, lead->cfg.sm_start, bh->work_cap};
     for (int i = 0; i < n; i++) {
@@ -539,12 +622,21 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     }
     if (!bh->gexec || key != bh->gkey) {
       invalidate_graph(bh);
-      hipGraph_t g;
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int c0 = refine_launches(lead, bh->b, niters, prop_mode, shape, s);
-      HIP_TRY(hipStreamEndCapture(s, &g));
-      HIP_TRY(hipGraphInstantiate(&bh->gexec, g, nullptr, nullptr, 0));
+      hipGraph_t g = nullptr;
+      const hipError_t le = hipGetLastError();
+      const hipError_t ee = hipStreamEndCapture(s, &g);  // always leave capture mode
+      if (le != hipSuccess || ee != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return STSP_ERR_HIP;
+      }
+      const hipError_t ie = hipGraphInstantiate(&bh->gexec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        bh->gexec = nullptr;
+        return STSP_ERR_HIP;
+      }
       bh->gkey = key;
       bh->gcur = c0;
     }
@@ -586,6 +678,9 @@ static int bass_prologue(stsp_ctx* c, const float* img, hipStream_t s) {
   c->nspix_prev = 0;
   c->prop_mode = 0;
   c->niters = cfg.niters;
+  c->split_alpha = cfg.split_alpha;
+  c->merge_alpha = 0.f;  // local to bass() (bass.cu:55)
+  c->thresh_new = cfg.thresh_new;
   c->prev_prior_mu_app = nullptr;
   // mark_active_contiguous + init_sp_params (init_sparams.cu:25-231)
   launch_bass_mark(c->P, nspix, nbuf, cfg.sp_size, s);
@@ -655,6 +750,9 @@ static int prop_prologue(stsp_ctx* c, const float* img, const float* flow, const
   c->nbuf = nbuf;
   c->nspix_prev = n;
   c->prop_mode = 1;
+  c->split_alpha = cfg.split_alpha;
+  c->merge_alpha = cfg.merge_alpha;
+  c->thresh_new = cfg.thresh_new;
   c->niters = cfg.prop_niters > 0 ? cfg.prop_niters : adapt;
   c->prev_prior_mu_app = prev->prior_mu_app;
   HIP_TRY(hipGetLastError());
@@ -719,6 +817,7 @@ void stsp_default_config(stsp_config* cfg) {
   cfg->merge_alpha = 0.0f;
   cfg->thresh_relabel = 1e-5f;
   cfg->thresh_new = 1e-2f;
+  cfg->target_nspix = 0;
 }
 
 const char* stsp_status_string(int status) {
@@ -850,31 +949,25 @@ int stsp_batch_use_graph(stsp_batch* bh, int on) {
   return STSP_OK;
 }
 
-int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
-                    const float** flows, const int** spix_prev, stsp_params* prevs,
-                    const int* nspix_prev, int** spix_out, stsp_params* outs, int* nspix_out,
-                    int* niters_state, hipStream_t s) {
-  if (!bh || !ctxs || n < 1 || n > bh->maxf || !imgs || !flows || !spix_prev || !prevs ||
-      !nspix_prev || !spix_out || !outs || !nspix_out)
-    return STSP_ERR_ARG;
-  for (int i = 0; i < n; i++) {
-    if (!ctxs[i] || nspix_prev[i] < 1) return STSP_ERR_ARG;
-    for (int j = 0; j < i; j++)
-      if (ctxs[j] == ctxs[i]) return STSP_ERR_ARG;
-  }
-  for (int i = 0; i < n; i++) {
-    int rc = prop_prologue(ctxs[i], imgs[i], flows[i], spix_prev[i], &prevs[i], nspix_prev[i],
-                           niters_state ? &niters_state[i] : nullptr, s);
-    if (rc) return rc;
-  }
-  // frames with the same schedule refine together
+}  // extern "C"
+
+// Refines frames whose prologues are done: frames with the same schedule
+// (iteration count, mode, sweeps per iteration, split/merge period and start)
+// share one launch sequence; then compaction, one wait, and the exports.
+static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_out,
+                             stsp_params* outs, int* nspix_out, hipStream_t s) {
   std::vector<int> done(n, 0);
   std::vector<stsp_ctx*> group;
   for (int i = 0; i < n; i++) {
     if (done[i]) continue;
     group.clear();
+    const stsp_ctx* a = ctxs[i];
     for (int j = i; j < n; j++) {
-      if (done[j] || ctxs[j]->niters != ctxs[i]->niters) continue;
+      const stsp_ctx* c = ctxs[j];
+      if (done[j] || c->niters != a->niters || c->prop_mode != a->prop_mode ||
+          c->cfg.niters_seg != a->cfg.niters_seg || c->cfg.sm_period != a->cfg.sm_period ||
+          c->cfg.sm_start != a->cfg.sm_start)
+        continue;
       group.push_back(ctxs[j]);
       done[j] = 1;
     }
@@ -892,6 +985,67 @@ int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
     if (rc) return rc;
   }
   return STSP_OK;
+}
+
+static int check_batch_ctxs(stsp_batch* bh, stsp_ctx** ctxs, int n) {
+  if (!bh || !ctxs || n < 1 || n > bh->maxf) return STSP_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || ctxs[i]->cfg.target_nspix > 0) return STSP_ERR_ARG;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return STSP_ERR_ARG;
+  }
+  return STSP_OK;
+}
+
+extern "C" {
+
+int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
+                    const float** flows, const int** spix_prev, stsp_params* prevs,
+                    const int* nspix_prev, int** spix_out, stsp_params* outs, int* nspix_out,
+                    int* niters_state, hipStream_t s) {
+  int rc = check_batch_ctxs(bh, ctxs, n);
+  if (rc) return rc;
+  if (!imgs || !flows || !spix_prev || !prevs || !nspix_prev || !spix_out || !outs || !nspix_out)
+    return STSP_ERR_ARG;
+  for (int i = 0; i < n; i++)
+    if (nspix_prev[i] < 1) return STSP_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    rc = prop_prologue(ctxs[i], imgs[i], flows[i], spix_prev[i], &prevs[i], nspix_prev[i],
+                       niters_state ? &niters_state[i] : nullptr, s);
+    if (rc) return rc;
+  }
+  return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
+}
+
+int stsp_bass_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs, int** spix_out,
+                    stsp_params* outs, int* nspix_out, hipStream_t s) {
+  int rc = check_batch_ctxs(bh, ctxs, n);
+  if (rc) return rc;
+  if (!imgs || !spix_out || !outs || !nspix_out) return STSP_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    rc = bass_prologue(ctxs[i], imgs[i], s);
+    if (rc) return rc;
+  }
+  return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
+}
+
+int stsp_rescale_image(const uint8_t* img_u8, float* out, int nimages, int npix, float scale,
+                       hipStream_t stream) {
+  if (!img_u8 || !out || nimages < 1 || npix < 1 || !(scale != 0.0f)) return STSP_ERR_ARG;
+  launch_rescale_u8(img_u8, out, 3ll * nimages * npix, scale, stream);
+  return hipGetLastError() == hipSuccess ? STSP_OK : STSP_ERR_HIP;
+}
+
+int stsp_rgb2lab(const float* img_bgr, float* img_lab, int nimages, int npix, hipStream_t stream) {
+  if (!img_bgr || !img_lab || nimages < 1 || npix < 1) return STSP_ERR_ARG;
+  launch_rgb_to_lab(img_bgr, img_lab, (long long)nimages * npix, stream);
+  return hipGetLastError() == hipSuccess ? STSP_OK : STSP_ERR_HIP;
+}
+
+int stsp_lab2rgb(const float* img_lab, float* img_rgb, int nimages, int npix, hipStream_t stream) {
+  if (!img_lab || !img_rgb || nimages < 1 || npix < 1) return STSP_ERR_ARG;
+  launch_lab_to_rgb(img_lab, img_rgb, (long long)nimages * npix, stream);
+  return hipGetLastError() == hipSuccess ? STSP_OK : STSP_ERR_HIP;
 }
 
 int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,

@@ -505,6 +505,19 @@ int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut
 #undef STSP_SWEEP
 }
 
+// Labels with at least one pixel after the last statistics pass, per frame,
+// added to counter[slot] (the distinct labels of the map).
+__global__ void kb_count_living(Batch b, int slot) {
+  LAB_PROLOGUE
+  const bool alive = k < F.nbuf && F.st.count[k] > 0;
+  const int n = __popcll(__ballot(alive));
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(F.counter + slot, n);
+}
+
+void b_count_living(const Batch& b, int slot, hipStream_t s) {
+  kb_count_living<<<b.lab_blocks, kBlock, 0, s>>>(b, slot);
+}
+
 // ---------------------------------------------------------------------------
 // per-frame helpers
 // ---------------------------------------------------------------------------

@@ -65,6 +65,11 @@ void launch_prior_update(const int* counts_prev, const int* counts_s, const int*
                          float* prior_counts, float* invalid_counts, int n, hipStream_t s);
 void launch_prop_params_init(Params P, InParams in, int nspix, int nbuf, int sp_size,
                              const int* filled, int npix, hipStream_t s);
+// colour preprocessing (k_color.hip)
+void launch_rescale_u8(const uint8_t* in, float* out, long long n, float scale, hipStream_t s);
+void launch_rgb_to_lab(const float* bgr, float* lab, long long npix, hipStream_t s);
+void launch_lab_to_rgb(const float* lab, float* rgb, long long npix, hipStream_t s);
+
 }  // namespace stsp
 
 namespace stsp {

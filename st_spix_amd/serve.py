@@ -139,11 +139,14 @@ class Server:
             (fresh if st.params is None else warm).append(st)
         done: List[Request] = []
         with torch.cuda.stream(self.cuda_stream):
-            for st in fresh:
-                r = st.queue.popleft()
-                st.spix, st.params = core.bass_frame(r.img, st.cfg, ctx=st.ctx)
-                st.frames += 1
-                done.append(r)
+            if fresh:
+                reqs = [st.queue.popleft() for st in fresh]
+                outs = self.runner.bass([(st.ctx, r.img) for st, r in zip(fresh, reqs)],
+                                        stream=self.cuda_stream)
+                for st, r, (spix, params) in zip(fresh, reqs, outs):
+                    st.spix, st.params = spix, params
+                    st.frames += 1
+                    done.append(r)
             if warm:
                 items = []
                 reqs = []

@@ -185,3 +185,49 @@ def test_prop_all_holes_raises():
     with pytest.raises(_lib.StspError) as e:
         core.prop_frame(cuda(img), cuda(flow), spix, params, cfg)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("target", [40, 120])
+def test_target_nspix_bass(orc, target):
+    """Label-count control (bass.cu:65-105): the alphas are steered every
+    second iteration until the count is within 5% of the target."""
+    from st_spix_amd import core
+    img = synth.voronoi_image(64, 80, 10, seed=3)
+    kw = dict(niters=10, sp_size=8, potts=10.0, target_nspix=target)
+    spix, params = core.bass_frame(cuda(img), core.make_config(**kw))
+    torch.cuda.synchronize()
+    seg_r, p_r = orc.bass(img, orc_cfg(orc, **kw))
+    assert_labels(spix, seg_r, f"target {target}")
+    assert_params(params, p_r, f"target {target}")
+    n = len(np.unique(seg_r))
+    assert 0.95 * target < n < 1.05 * target
+
+
+def test_target_nspix_prop(orc):
+    """The control in propagated frames (prop.cu:356-401)."""
+    from st_spix_amd import core
+    vid, flows = synth.voronoi_video(3, 64, 80, nregions=10, vmax=3.0, seed=11)
+    kw = dict(niters=10, sp_size=8, potts=10.0, target_nspix=60)
+    cfg, ocfg = core.make_config(**kw), orc_cfg(orc, **kw)
+    spix, params = core.bass_frame(cuda(vid[0]), cfg)
+    seg_r, p_r = orc.bass(vid[0], ocfg)
+    for t in range(1, 3):
+        spix, params, ns = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg)
+        seg_r, p_r, ns_r = orc.prop(vid[t], flows[t - 1], seg_r, p_r, ocfg)
+        torch.cuda.synchronize()
+        assert ns == ns_r
+        assert_labels(spix, seg_r, f"frame{t}")
+        assert_params(params, p_r, f"frame{t}")
+
+
+def test_batch_rejects_target_mode():
+    """The label-count control is a single-frame mode; batches refuse it."""
+    from st_spix_amd import _lib, core
+    ctx = core.Context(32, 40, core.make_config(sp_size=8, niters=4, target_nspix=10))
+    runner = core.BatchRunner(2)
+    img = cuda(synth.voronoi_image(32, 40, 4, seed=1))
+    with pytest.raises(_lib.StspError) as e:
+        runner.bass([(ctx, img)])
+    assert e.value.code == -1
+    runner.close()
+    ctx.close()

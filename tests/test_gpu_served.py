@@ -105,3 +105,52 @@ def test_server_admission_rejects():
         for _ in range(100):
             srv.open_stream(1080, 1920)
     srv.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_bass_batch_matches_oracle(orc, use_graph):
+    """Config 2's form: images of different sizes refined in shared launches
+    (stsp_bass_batch); each equals the oracle's BASS of that image alone.
+    Sizes are drawn like config 2's (H in [240, 720], W in [320, 1280]) but
+    scaled by 1/8 so the oracle finishes in seconds; two calls in a row
+    replay the same batch composition."""
+    rng = np.random.default_rng(7)
+    sizes = [(int(rng.integers(30, 91)), int(rng.integers(40, 161))) for _ in range(8)]
+    runner = core.BatchRunner(8, use_graph=use_graph)
+    ctxs = [core.Context(H, W, core.make_config(sp_size=6, niters=16, potts=10.0))
+            for H, W in sizes]
+    side = torch.cuda.Stream()  # graphs are captured on a non-default stream
+    for rep in range(2):
+        imgs = [synth.voronoi_image(H, W, 6, seed=100 * rep + i) for i, (H, W) in enumerate(sizes)]
+        with torch.cuda.stream(side):
+            outs = runner.bass([(c, cuda(im)) for c, im in zip(ctxs, imgs)], stream=side)
+        side.synchronize()
+        junk = torch.randn(1 << 16, device="cuda").sum()  # other work between replays
+        torch.cuda.synchronize()
+        del junk
+        for (H, W), im, (spix, params) in zip(sizes, imgs, outs):
+            seg_r, p_r = orc.bass(im, orc_cfg(orc, sp_size=6, niters=16, potts=10.0))
+            assert_labels(spix, seg_r, f"rep{rep} {H}x{W}")
+            assert_params(params, p_r, f"rep{rep} {H}x{W}")
+    runner.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_bass_batch_config2_sizes(orc):
+    """Two images at config 2's full sizes (sp 20) in one batch, with graph
+    replay requested on the default stream (it runs eagerly there)."""
+    sizes = [(240, 320), (720, 1280)]
+    runner = core.BatchRunner(4, use_graph=True)
+    kw = dict(sp_size=20, niters=12, potts=0.5)
+    ctxs = [core.Context(H, W, core.make_config(**kw)) for H, W in sizes]
+    imgs = [synth.voronoi_image(H, W, 30, seed=i) for i, (H, W) in enumerate(sizes)]
+    outs = runner.bass([(c, cuda(im)) for c, im in zip(ctxs, imgs)])
+    torch.cuda.synchronize()
+    for (H, W), im, (spix, params) in zip(sizes, imgs, outs):
+        seg_r, p_r = orc.bass(im, orc_cfg(orc, **kw))
+        assert_labels(spix, seg_r, f"{H}x{W}")
+        assert_params(params, p_r, f"{H}x{W}")
+    runner.close()
+    for c in ctxs:
+        c.close()

@@ -270,3 +270,14 @@ def test_prop_zero_flow_static_video(orc):
     assert ns == 4  # < 1% invalid -> 4 iterations (pyapi.cu:141-149)
     np.testing.assert_array_equal(p0["invalid_counts"], 0)
     assert seg1.min() >= 0
+
+
+@pytest.mark.parametrize("target", [40, 120])
+def test_target_nspix_converges(orc, target):
+    """With a target the oracle's BASS ends within 5% of it (bass.cu:96-98),
+    from above and from below the hexagonal start (K0 = 48 here)."""
+    img = synth.voronoi_image(64, 80, 10, seed=3)
+    seg, p = orc.bass(img, orc.make_cfg(niters=10, sp_size=8, potts=10.0, target_nspix=target))
+    n = len(np.unique(seg))
+    assert 0.95 * target < n < 1.05 * target
+    assert len(p["ids"]) == n
