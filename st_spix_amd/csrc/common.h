@@ -35,7 +35,8 @@ struct alignas(16) SweepRec {
 };
 
 // Per-label sufficient statistics.  S = colour sums, Q = colour square sums
-// (2^-32 fixed point), positions exact.
+// (2^-32 fixed point), positions exact.  Q is gathered only for the split and
+// merge statistics; nothing reads it for the live labels.
 struct LabelStats {
   int* count;    // [cap]
   int64_t* S;    // [cap*3]

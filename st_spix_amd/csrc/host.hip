@@ -509,6 +509,19 @@ static void iteration_tail(stsp_ctx* lead, const Batch& b, int idx, int prop_mod
   tstop(lead, STSP_KCLASS_SWEEP, s);
 }
 
+// The parameter update at the start of an iteration.  The sweeps keep the
+// label statistics current (a changed pixel moves its integer sums), so a
+// full recount is needed only at the first iteration (the prologue leaves
+// no statistics) and before a split (it also gathers the bounding boxes);
+// the updates after split and merge recount as well.
+static void update_at_start(const Batch& b, const stsp_config& cfg, int idx, int cur,
+                            hipStream_t s) {
+  if (idx == 0 || is_split_iter(cfg, idx))
+    b_update_params(b, cur, is_split_iter(cfg, idx), s);
+  else
+    b_param_only(b, s);
+}
+
 // The refinement loop for every frame of the batch; returns the live
 // buffer index.
 static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_mode,
@@ -518,7 +531,7 @@ static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_
   int smc = 1;  // split/merge counter (bass.cu:56, prop.cu:343)
   for (int idx = 0; idx < niters; idx++) {
     tstart(lead, STSP_KCLASS_STATS, s);
-    b_update_params(b, cur, is_split_iter(cfg, idx), s);
+    update_at_start(b, cfg, idx, cur, s);
     tstop(lead, STSP_KCLASS_STATS, s);
     iteration_tail(lead, b, idx, prop_mode, shape, &cur, &smc, s);
   }
@@ -569,7 +582,7 @@ static int refine_controlled(stsp_batch* bh, stsp_ctx* c, hipStream_t s) {
   const int og_niters = c->niters;
   int cur = 0, smc = 1;
   for (int idx = 0; idx < 5000; idx++) {
-    b_update_params(bh->b, cur, is_split_iter(cfg, idx), s);
+    update_at_start(bh->b, cfg, idx, cur, s);
     if (idx % 2 == 0) {
       k_set_int<<<1, 1, 0, s>>>(c->counter + 1, 0);
       b_count_living(bh->b, 1, s);
@@ -659,102 +672,149 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // prologues and epilogue
 // ---------------------------------------------------------------------------
-static int bass_prologue(stsp_ctx* c, const float* img, hipStream_t s) {
-  const stsp_config& cfg = c->cfg;
-  int H = c->H, W = c->W, npix = c->npix;
-  HexGeom g = hex_geometry(cfg.sp_size, W, H);
-  int K0 = g.nx * g.ny;
-  int nbuf = 30 * K0;  // bass.cu:245
-  int rc = ensure_labels(c, nbuf);
-  if (rc) return rc;
-  launch_hex_init(c->seg, H, W, cfg.sp_size, s);
-  launch_compact(c->seg, npix, nullptr, K0, 0, nbuf, c->alive, c->flags, c->rank, c->num_new,
-                 c->ctmp, s);
-  int nspix = 0;
-  rc = read_int(c, c->num_new, s, &nspix);
-  if (rc) return rc;
-  c->img = img;
-  c->nbuf = nbuf;
-  c->nspix_prev = 0;
-  c->prop_mode = 0;
-  c->niters = cfg.niters;
-  c->split_alpha = cfg.split_alpha;
-  c->merge_alpha = 0.f;  // local to bass() (bass.cu:55)
-  c->thresh_new = cfg.thresh_new;
-  c->prev_prior_mu_app = nullptr;
-  // mark_active_contiguous + init_sp_params (init_sparams.cu:25-231)
-  launch_bass_mark(c->P, nspix, nbuf, cfg.sp_size, s);
-  stsp_ctx* one = c;
-  rc = batch_build(&c->self, &one, 1, s);
-  if (rc) return rc;
-  b_param_summary(c->self.b, 0, s);
-  launch_bass_init(c->P, nspix, nbuf, cfg.sp_size, npix, s);
-  k_set_int<<<1, 1, 0, s>>>(c->max_spix, nspix - 1);
+// First frames (BASS set-up, bass.cu:178-330) of frames cs[0..n): the
+// hexagonal initialisation and compaction of every frame, one wait for all
+// label counts, then the parameter set-up.
+static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, hipStream_t s) {
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    const stsp_config& cfg = c->cfg;
+    HexGeom g = hex_geometry(cfg.sp_size, c->W, c->H);
+    int K0 = g.nx * g.ny;
+    int nbuf = 30 * K0;  // bass.cu:245
+    int rc = ensure_labels(c, nbuf);
+    if (rc) return rc;
+    c->nbuf = nbuf;
+    launch_hex_init(c->seg, c->H, c->W, cfg.sp_size, s);
+    launch_compact(c->seg, c->npix, nullptr, K0, 0, nbuf, c->alive, c->flags, c->rank,
+                   c->num_new, c->ctmp, s);
+    HIP_TRY(hipMemcpyAsync(c->host, c->num_new, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    const stsp_config& cfg = c->cfg;
+    const int nspix = c->host[0];
+    c->img = imgs[i];
+    c->nspix_prev = 0;
+    c->prop_mode = 0;
+    c->niters = cfg.niters;
+    c->split_alpha = cfg.split_alpha;
+    c->merge_alpha = 0.f;  // local to bass() (bass.cu:55)
+    c->thresh_new = cfg.thresh_new;
+    c->prev_prior_mu_app = nullptr;
+    // mark_active_contiguous + init_sp_params (init_sparams.cu:25-231)
+    launch_bass_mark(c->P, nspix, c->nbuf, cfg.sp_size, s);
+    stsp_ctx* one = c;
+    int rc = batch_build(&c->self, &one, 1, s);
+    if (rc) return rc;
+    b_param_summary(c->self.b, 0, s);
+    launch_bass_init(c->P, nspix, c->nbuf, cfg.sp_size, c->npix, s);
+    k_set_int<<<1, 1, 0, s>>>(c->max_spix, nspix - 1);
+  }
   HIP_TRY(hipGetLastError());
   return STSP_OK;
 }
 
-static int fill_loop(stsp_ctx* c, hipStream_t s) {
-  int prev_neg = -1;
+// Hole filling (fill_missing.cu:35-130) of frames cs[0..n) together: every
+// round counts the holes of each unfinished frame, one wait reads all the
+// counts, and each frame that still has holes gets one round of the four
+// fill phases.  Per frame these are exactly the reference's rounds, with
+// its stop rule (no holes left) and its failure (the count stalls).
+static int fill_loop_n(stsp_ctx** cs, int n, hipStream_t s) {
+  std::vector<int> prev(n, -1), live(n, 1);
   for (int iter = 0; iter < 1000; iter++) {
-    k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
-    launch_fill_count(c->seg, c->border, c->num_neg, c->W, c->H, s);
-    int neg = 0;
-    int rc = read_int(c, c->num_neg, s, &neg);
-    if (rc) return rc;
-    if (neg == 0) return STSP_OK;
-    if (iter > 0 && neg == prev_neg) return STSP_ERR_FILL;
-    launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, s);
-    prev_neg = neg;
+    int any = 0;
+    for (int i = 0; i < n; i++) {
+      if (!live[i]) continue;
+      stsp_ctx* c = cs[i];
+      k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
+      launch_fill_count(c->seg, c->border, c->num_neg, c->W, c->H, s);
+      HIP_TRY(hipMemcpyAsync(c->host + 2, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, s));
+      any = 1;
+    }
+    if (!any) return STSP_OK;
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) {
+      if (!live[i]) continue;
+      stsp_ctx* c = cs[i];
+      const int neg = c->host[2];
+      if (neg == 0) {
+        live[i] = 0;
+        continue;
+      }
+      if (iter > 0 && neg == prev[i]) return STSP_ERR_FILL;
+      launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, s);
+      prev[i] = neg;
+    }
   }
   return STSP_OK;
 }
 
-// shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up (prop.cu:677-745)
-static int prop_prologue(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
-                         stsp_params* prev, int n, int* niters_state, hipStream_t s) {
-  const stsp_config& cfg = c->cfg;
-  int H = c->H, W = c->W, npix = c->npix;
-  int nbuf = 10 * n;  // prop.cu:702
-  int rc = ensure_labels(c, nbuf);
+// shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up
+// (prop.cu:677-745) for frames cs[0..n), stage by stage across frames so
+// that the host waits (hole-filling rounds, invalid counts) are shared.
+static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
+                           const float* const* flows, const int* const* spix_prev,
+                           stsp_params* prevs, const int* nprev, int* niters_state,
+                           hipStream_t s) {
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    const int np = nprev[i];
+    int rc = ensure_labels(c, 10 * np);  // prop.cu:702
+    if (rc) return rc;
+    rc = ensure_prev(c, np);
+    if (rc) return rc;
+    launch_flow_pool(flows[i], spix_prev[i], c->npix, c->fsum, c->counts_prev,
+                     prevs[i].mu_shape, c->flow_ds, c->centers, np, s);
+    launch_shift(spix_prev[i], c->flow_ds, c->counts_prev, c->keys, c->shifted, c->W, c->H, s);
+    HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * c->npix, hipMemcpyDeviceToDevice,
+                           s));
+    launch_count_labels(c->shifted, c->npix, c->counts_s, np, s);
+  }
+  int rc = fill_loop_n(cs, n, s);
   if (rc) return rc;
-  rc = ensure_prev(c, n);
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, c->W, c->H, nprev[i], s);
+  }
+  rc = fill_loop_n(cs, n, s);
   if (rc) return rc;
-  launch_flow_pool(flow, spix_prev, npix, c->fsum, c->counts_prev, prev->mu_shape, c->flow_ds,
-                   c->centers, n, s);
-  launch_shift(spix_prev, c->flow_ds, c->counts_prev, c->keys, c->shifted, W, H, s);
-  HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * npix, hipMemcpyDeviceToDevice, s));
-  launch_count_labels(c->shifted, npix, c->counts_s, n, s);
-  rc = fill_loop(c, s);
-  if (rc) return rc;
-  launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, W, H, n, s);
-  rc = fill_loop(c, s);
-  if (rc) return rc;
-  launch_count_labels(c->seg, npix, c->counts_f, n, s);
-  launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prev->prior_counts,
-                      prev->invalid_counts, n, s);
-  // adaptive iteration count (pyapi.cu:139-149)
-  k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
-  launch_fill_count(c->shifted, c->border, c->num_neg, W, H, s);
-  int ninvalid = 0;
-  rc = read_int(c, c->num_neg, s, &ninvalid);
-  if (rc) return rc;
-  float iperc = (float)(ninvalid / (1.0 * npix));
-  int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
-  if (niters_state) *niters_state = adapt;
-  InParams in;
-  memcpy(&in, prev, sizeof(in));
-  launch_prop_params_init(c->P, in, n, nbuf, cfg.sp_size, c->seg, npix, s);
-  k_set_int<<<1, 1, 0, s>>>(c->max_spix, n - 1);
-  c->img = img;
-  c->nbuf = nbuf;
-  c->nspix_prev = n;
-  c->prop_mode = 1;
-  c->split_alpha = cfg.split_alpha;
-  c->merge_alpha = cfg.merge_alpha;
-  c->thresh_new = cfg.thresh_new;
-  c->niters = cfg.prop_niters > 0 ? cfg.prop_niters : adapt;
-  c->prev_prior_mu_app = prev->prior_mu_app;
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    const int np = nprev[i];
+    launch_count_labels(c->seg, c->npix, c->counts_f, np, s);
+    launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prevs[i].prior_counts,
+                        prevs[i].invalid_counts, np, s);
+    // adaptive iteration count (pyapi.cu:139-149): holes of the shifted map
+    k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
+    launch_fill_count(c->shifted, c->border, c->num_neg, c->W, c->H, s);
+    HIP_TRY(hipMemcpyAsync(c->host + 3, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = cs[i];
+    const stsp_config& cfg = c->cfg;
+    const int np = nprev[i];
+    const int nbuf = 10 * np;
+    const int ninvalid = c->host[3];
+    float iperc = (float)(ninvalid / (1.0 * c->npix));
+    int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
+    if (niters_state) niters_state[i] = adapt;
+    InParams in;
+    memcpy(&in, &prevs[i], sizeof(in));
+    launch_prop_params_init(c->P, in, np, nbuf, cfg.sp_size, c->seg, c->npix, s);
+    k_set_int<<<1, 1, 0, s>>>(c->max_spix, np - 1);
+    c->img = imgs[i];
+    c->nbuf = nbuf;
+    c->nspix_prev = np;
+    c->prop_mode = 1;
+    c->split_alpha = cfg.split_alpha;
+    c->merge_alpha = cfg.merge_alpha;
+    c->thresh_new = cfg.thresh_new;
+    c->niters = cfg.prop_niters > 0 ? cfg.prop_niters : adapt;
+    c->prev_prior_mu_app = prevs[i].prior_mu_app;
+  }
   HIP_TRY(hipGetLastError());
   return STSP_OK;
 }
@@ -906,7 +966,8 @@ int stsp_ctx_kernel_time(stsp_ctx* c, int kclass, double* total_ms, long long* l
 int stsp_bass(stsp_ctx* c, const float* img, int* spix_out, stsp_params* out, int* nspix_out,
               hipStream_t s) {
   if (!c || !img || !spix_out || !out || !nspix_out) return STSP_ERR_ARG;
-  int rc = bass_prologue(c, img, s);
+  const float* im = img;
+  int rc = bass_prologue_n(&c, &im, 1, s);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
 }
@@ -917,7 +978,7 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
   if (!c || !img || !flow || !spix_prev || !prev || !spix_out || !out || !nspix_out ||
       nspix_prev < 1)
     return STSP_ERR_ARG;
-  int rc = prop_prologue(c, img, flow, spix_prev, prev, nspix_prev, niters_state, s);
+  int rc = prop_prologue_n(&c, 1, &img, &flow, &spix_prev, prev, &nspix_prev, niters_state, s);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
 }
@@ -1009,11 +1070,8 @@ int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
     return STSP_ERR_ARG;
   for (int i = 0; i < n; i++)
     if (nspix_prev[i] < 1) return STSP_ERR_ARG;
-  for (int i = 0; i < n; i++) {
-    rc = prop_prologue(ctxs[i], imgs[i], flows[i], spix_prev[i], &prevs[i], nspix_prev[i],
-                       niters_state ? &niters_state[i] : nullptr, s);
-    if (rc) return rc;
-  }
+  rc = prop_prologue_n(ctxs, n, imgs, flows, spix_prev, prevs, nspix_prev, niters_state, s);
+  if (rc) return rc;
   return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
 }
 
@@ -1022,10 +1080,8 @@ int stsp_bass_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs, 
   int rc = check_batch_ctxs(bh, ctxs, n);
   if (rc) return rc;
   if (!imgs || !spix_out || !outs || !nspix_out) return STSP_ERR_ARG;
-  for (int i = 0; i < n; i++) {
-    rc = bass_prologue(ctxs[i], imgs[i], s);
-    if (rc) return rc;
-  }
+  rc = bass_prologue_n(ctxs, imgs, n, s);
+  if (rc) return rc;
   return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
 }
 

@@ -83,7 +83,9 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 // atomics.  Position moments are summed relative to the tile origin in int32
 // and widened exactly: sum x = n*x0 + sum dx, sum x^2 = n*x0^2 + 2*x0*sum dx
 // + sum dx^2, sum xy = n*x0*y0 + x0*sum dy + y0*sum dx + sum dx*dy.
-template <bool BBOX>
+// SQ: also gather the colour squares (only the split and merge passes read
+// them; the live-label statistics never do).
+template <bool BBOX, bool SQ>
 __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int which) {
   const int f = frame_of(b.off.st, b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           S[c] += fx_color(col[j][c]);
-          Q[c] += fx_square(col[j][c]);
+          if (SQ) Q[c] += fx_square(col[j][c]);
         }
         if (BBOX) {
           mnx = min(mnx, tx0 + dx);
@@ -165,7 +167,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       S[c] = wave_sum64_to63(S[c]);
-      Q[c] = wave_sum64_to63(Q[c]);
+      if (SQ) Q[c] = wave_sum64_to63(Q[c]);
     }
     if (BBOX) {
       mnx = wave_min_to63(mnx);
@@ -180,7 +182,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         atomic_add_i64(st.S + 3 * u + c, S[c]);
-        atomic_add_i64(st.Q + 3 * u + c, Q[c]);
+        if (SQ) atomic_add_i64(st.Q + 3 * u + c, Q[c]);
       }
       int64_t* X = st.X + 5 * u;
       atomic_add_i64(X + 0, n * X0 + sdx);
@@ -319,18 +321,23 @@ __global__ void kb_store_sample(Batch b) {
   }
 }
 
+// parameters from the statistics as they stand (kept current by the sweeps)
+void b_param_only(const Batch& b, hipStream_t s) {
+  kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
+}
+
 void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, bbox ? 1 : 0);
   if (bbox)
-    kb_stats_accum<true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<true, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   else
-    kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
 void b_param_summary(const Batch& b, int cur, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
-  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+  kb_stats_accum<false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 1);
 }
 
@@ -473,10 +480,35 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
       __syncthreads();
     }
   }
+  // write the tile; a pixel that changed label moves its exact integer
+  // statistics from the old label to the new one, so the label statistics
+  // stay equal to a full recount (integer sums are order-free)
+  LabelStats st = F.st;
   for (int i = tid; i < TW * TH; i += NT) {
     int lx = R + i % TW, ly = R + i / TW;
     int gx = x0 + lx, gy = y0 + ly;
-    if (gx < W && gy < H) seg_out[gy * W + gx] = lab[ly * LW + lx];
+    if (gx >= W || gy >= H) continue;
+    const int g = gy * W + gx;
+    const int nl = lab[ly * LW + lx];
+    seg_out[g] = nl;
+    const int ol = seg_in[g];
+    if (nl == ol) continue;
+    const int64_t x = gx, y = gy;
+    // (the live-label colour squares are never read, so they are not kept)
+    const int64_t v[8] = {fx_color(img[3 * g]), fx_color(img[3 * g + 1]), fx_color(img[3 * g + 2]),
+                          x, y, x * x, x * y, y * y};
+    atomicAdd(st.count + ol, -1);
+    atomicAdd(st.count + nl, 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      atomic_add_i64(st.S + 3 * ol + c, -v[c]);
+      atomic_add_i64(st.S + 3 * nl + c, v[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      atomic_add_i64(st.X + 5 * ol + c, -v[3 + c]);
+      atomic_add_i64(st.X + 5 * nl + c, v[3 + c]);
+    }
   }
 }
 
@@ -606,9 +638,15 @@ __global__ void kb_split_list(Batch b) {
   const int* bb = F.st.bbox + 4 * k;
   const int bw = bb[1] - bb[0] + 1, bh = bb[3] - bb[2] + 1;
   if (bw < 1 || bh < 1) return;
-  const bool small = bw <= kWaveBox && bh <= kWaveBox;
-  const int pos = atomicAdd(b.work_n + (small ? 0 : 2), 1);
-  if (pos < b.work_cap) b.work[(small ? 0 : b.work_cap) + pos] = (f << 24) | k;
+  // one atomic per list with a wave-uniform address, so the compiler
+  // aggregates each wave's increments into one
+  if (bw <= kWaveBox && bh <= kWaveBox) {
+    const int pos = atomicAdd(b.work_n + 0, 1);
+    if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
+  } else {
+    const int pos = atomicAdd(b.work_n + 2, 1);
+    if (pos < b.work_cap) b.work[b.work_cap + pos] = (f << 24) | k;
+  }
 }
 
 __device__ __forceinline__ int seed_of(const FrameDesc& F, const int* seg, int k, int dir) {
@@ -1034,7 +1072,7 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
   kb_bfs_block<<<256, 256, 0, s>>>(b, cur, direction);
-  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
+  kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
   kb_split_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
@@ -1185,7 +1223,7 @@ void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
-  kb_stats_accum<false><<<b.st_blocks, 256, 0, s>>>(b, cur, 2);
+  kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 2);
   kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_marg<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
