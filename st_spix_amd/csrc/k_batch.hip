@@ -415,7 +415,8 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     int gx = x0 + lx, gy = y0 + ly;
     lab[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? seg_in[gy * W + gx] : -1;
   }
-  for (int s = 0; s < S; s++) {
+#pragma unroll
+  for (int s = 0; s < S; s++) {  // unrolled: region widths are constants
     if (tid < 4) cnt[tid] = 0;
     __syncthreads();
     {

@@ -108,12 +108,11 @@ def voronoi_video(T: int, H: int, W: int, nregions: int = 120, vmax: float = 4.0
             redo = out | (rng.uniform(0, 1, nregions) < churn)
             n = int(redo.sum())
             if n:
+                # re-enter on a random border side: 0 left, 1 right, 2 top, 3 bottom
                 side = rng.integers(0, 4, n)
                 pos = rng.uniform(0, 1, n)
-                sx = np.where(side == 0, 0.0, np.where(side == 1, W - 1.0, pos * W))
-                sy = np.where(side == 2, 0.0, np.where(side == 3, H - 1.0, pos * H))
-                sx = np.where(side < 2, sx, sx)
-                sy = np.where(side < 2, pos * H, sy)
+                sx = np.select([side == 0, side == 1], [0.0, W - 1.0], pos * W)
+                sy = np.select([side == 2, side == 3], [0.0, H - 1.0], pos * H)
                 seeds[redo] = np.stack([sx, sy], -1)
                 cols[redo] = random_colors(rng, n)
                 ang = rng.uniform(0, 2 * np.pi, n)

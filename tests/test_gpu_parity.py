@@ -231,3 +231,24 @@ def test_batch_rejects_target_mode():
     assert e.value.code == -1
     runner.close()
     ctx.close()
+
+
+def test_config4_full_size_sm_every_2(orc):
+    """Config 4's shape: 1080x1920, 400 regions entering and leaving, sp 15,
+    split/merge every 2 iterations (sm_period 2), BASS then two propagated
+    frames (6 iterations each, so the oracle finishes in seconds)."""
+    from st_spix_amd import core
+    vid, flows = synth.voronoi_video(3, 1080, 1920, nregions=400, vmax=4.0, churn=0.05, seed=4)
+    kw = dict(niters=6, sp_size=15, potts=0.5, sm_period=2, prop_niters=6)
+    cfg, ocfg = core.make_config(**kw), orc_cfg(orc, **kw)
+    spix, params = core.bass_frame(cuda(vid[0]), cfg)
+    seg_r, p_r = orc.bass(vid[0], ocfg)
+    assert_labels(spix, seg_r, "frame0")
+    assert_params(params, p_r, "frame0")
+    for t in range(1, 3):
+        spix, params, ns = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg)
+        seg_r, p_r, ns_r = orc.prop(vid[t], flows[t - 1], seg_r, p_r, ocfg)
+        torch.cuda.synchronize()
+        assert ns == ns_r
+        assert_labels(spix, seg_r, f"frame{t}")
+        assert_params(params, p_r, f"frame{t}")
