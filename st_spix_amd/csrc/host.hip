@@ -94,6 +94,20 @@ struct KTimer {
 
 }  // namespace
 
+// Per-frame work of a batch (prologues, compaction, export) is independent
+// across frames: it is spread over up to kLanes extra streams forked from the
+// caller's stream with an event and joined back with events, so the many
+// small launches of different frames overlap.  All streams have the default
+// priority.  With lanes == 0 everything runs on the caller's stream.
+constexpr int kLanes = 4;
+struct Lanes {
+  hipStream_t st[kLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kLanes] = {};
+  int created = 0;
+  int n = 0;
+};
+
 // A batch table on the device plus the cached refinement graph.
 struct stsp_batch {
   int maxf = 0;
@@ -109,6 +123,7 @@ struct stsp_batch {
   hipGraphExec_t gexec = nullptr;
   std::vector<long long> gkey;
   int gcur = 0;
+  Lanes lanes;
 };
 
 struct stsp_ctx {
@@ -138,6 +153,7 @@ struct stsp_ctx {
   unsigned long long* best;
   // scalars
   int *max_spix, *total, *num_new, *num_neg, *counter;
+  int* fillcnt;  // hole counts of a chunk of filling rounds (kFillChunk)
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
@@ -176,6 +192,7 @@ static int alloc_pixels(stsp_ctx* c) {
   c->num_new = a.take<int>(1);
   c->num_neg = a.take<int>(1);
   c->counter = a.take<int>(4);
+  c->fillcnt = a.take<int>(16);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
 }
@@ -306,7 +323,61 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   return STSP_OK;
 }
 
+static int lanes_fork(Lanes* L, hipStream_t s, int nframes) {
+  if (!L) return STSP_OK;
+  L->n = 0;
+  if (nframes < 2) return STSP_OK;
+  if (!L->created) {
+    for (int i = 0; i < kLanes; i++) {
+      HIP_TRY(hipStreamCreateWithFlags(&L->st[i], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&L->join[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&L->fork, hipEventDisableTiming));
+    L->created = 1;
+  }
+  L->n = nframes < kLanes ? nframes : kLanes;
+  HIP_TRY(hipEventRecord(L->fork, s));
+  for (int i = 0; i < L->n; i++) HIP_TRY(hipStreamWaitEvent(L->st[i], L->fork, 0));
+  return STSP_OK;
+}
+
+static inline hipStream_t lane_of(const Lanes* L, hipStream_t s, int frame) {
+  return (L && L->n) ? L->st[frame % L->n] : s;
+}
+
+// host wait for everything issued on the lanes (or on s without lanes)
+static int lanes_sync(const Lanes* L, hipStream_t s) {
+  if (!L || !L->n) {
+    HIP_TRY(hipStreamSynchronize(s));
+    return STSP_OK;
+  }
+  for (int i = 0; i < L->n; i++) HIP_TRY(hipStreamSynchronize(L->st[i]));
+  return STSP_OK;
+}
+
+static int lanes_join(Lanes* L, hipStream_t s) {
+  if (!L || !L->n) return STSP_OK;
+  for (int i = 0; i < L->n; i++) {
+    HIP_TRY(hipEventRecord(L->join[i], L->st[i]));
+    HIP_TRY(hipStreamWaitEvent(s, L->join[i], 0));
+  }
+  L->n = 0;
+  return STSP_OK;
+}
+
+static void lanes_free(Lanes* L) {
+  if (!L->created) return;
+  for (int i = 0; i < kLanes; i++) {
+    (void)hipStreamSynchronize(L->st[i]);
+    (void)hipStreamDestroy(L->st[i]);
+    (void)hipEventDestroy(L->join[i]);
+  }
+  (void)hipEventDestroy(L->fork);
+  L->created = 0;
+}
+
 static void batch_free(stsp_batch* bh) {
+  lanes_free(&bh->lanes);
   invalidate_graph(bh);
   if (bh->d_fr) (void)hipFree(bh->d_fr);
   if (bh->d_off) (void)hipFree(bh->d_off);
@@ -674,25 +745,31 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // First frames (BASS set-up, bass.cu:178-330) of frames cs[0..n): the
 // hexagonal initialisation and compaction of every frame, one wait for all
-// label counts, then the parameter set-up.
-static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, hipStream_t s) {
+// label counts, then the parameter set-up.  Frame i runs on lane i.
+static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, Lanes* L,
+                           hipStream_t s) {
+  int rc = lanes_fork(L, s, n);
+  if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
+    const hipStream_t ss = lane_of(L, s, i);
     const stsp_config& cfg = c->cfg;
     HexGeom g = hex_geometry(cfg.sp_size, c->W, c->H);
     int K0 = g.nx * g.ny;
     int nbuf = 30 * K0;  // bass.cu:245
-    int rc = ensure_labels(c, nbuf);
+    rc = ensure_labels(c, nbuf);
     if (rc) return rc;
     c->nbuf = nbuf;
-    launch_hex_init(c->seg, c->H, c->W, cfg.sp_size, s);
+    launch_hex_init(c->seg, c->H, c->W, cfg.sp_size, ss);
     launch_compact(c->seg, c->npix, nullptr, K0, 0, nbuf, c->alive, c->flags, c->rank,
-                   c->num_new, c->ctmp, s);
-    HIP_TRY(hipMemcpyAsync(c->host, c->num_new, sizeof(int), hipMemcpyDeviceToHost, s));
+                   c->num_new, c->ctmp, ss);
+    HIP_TRY(hipMemcpyAsync(c->host, c->num_new, sizeof(int), hipMemcpyDeviceToHost, ss));
   }
-  HIP_TRY(hipStreamSynchronize(s));
+  rc = lanes_sync(L, s);
+  if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
+    const hipStream_t ss = lane_of(L, s, i);
     const stsp_config& cfg = c->cfg;
     const int nspix = c->host[0];
     c->img = imgs[i];
@@ -704,48 +781,60 @@ static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, hipSt
     c->thresh_new = cfg.thresh_new;
     c->prev_prior_mu_app = nullptr;
     // mark_active_contiguous + init_sp_params (init_sparams.cu:25-231)
-    launch_bass_mark(c->P, nspix, c->nbuf, cfg.sp_size, s);
+    launch_bass_mark(c->P, nspix, c->nbuf, cfg.sp_size, ss);
     stsp_ctx* one = c;
-    int rc = batch_build(&c->self, &one, 1, s);
+    rc = batch_build(&c->self, &one, 1, ss);
     if (rc) return rc;
-    b_param_summary(c->self.b, 0, s);
-    launch_bass_init(c->P, nspix, c->nbuf, cfg.sp_size, c->npix, s);
-    k_set_int<<<1, 1, 0, s>>>(c->max_spix, nspix - 1);
+    b_param_summary(c->self.b, 0, ss);
+    launch_bass_init(c->P, nspix, c->nbuf, cfg.sp_size, c->npix, ss);
+    k_set_int<<<1, 1, 0, ss>>>(c->max_spix, nspix - 1);
   }
   HIP_TRY(hipGetLastError());
-  return STSP_OK;
+  return lanes_join(L, s);
 }
 
-// Hole filling (fill_missing.cu:35-130) of frames cs[0..n) together: every
-// round counts the holes of each unfinished frame, one wait reads all the
-// counts, and each frame that still has holes gets one round of the four
-// fill phases.  Per frame these are exactly the reference's rounds, with
-// its stop rule (no holes left) and its failure (the count stalls).
-static int fill_loop_n(stsp_ctx** cs, int n, hipStream_t s) {
+// Hole filling (fill_missing.cu:35-130) of frames cs[0..n) together.  The
+// reference's loop is: count the holes; stop at zero; fail if the count did
+// not shrink; else one round of the four fill phases.  Here each frame runs a
+// chunk of such rounds without waiting (the count of round r goes to
+// fillcnt[r]), and one wait reads all the counts.  Rounds after the count
+// reached zero change nothing (no hole is left to fill), and a stall is
+// found in the recorded counts, so the outcome is the reference's.
+constexpr int kFillChunk = 8;
+static int fill_loop_n(stsp_ctx** cs, int n, Lanes* L, hipStream_t s) {
   std::vector<int> prev(n, -1), live(n, 1);
-  for (int iter = 0; iter < 1000; iter++) {
+  int chunk = 4;  // most frames finish within a few rounds
+  for (int base = 0; base < 1000; base += chunk, chunk = kFillChunk) {
+    const int rounds = 1000 - base < chunk ? 1000 - base : chunk;
     int any = 0;
     for (int i = 0; i < n; i++) {
       if (!live[i]) continue;
       stsp_ctx* c = cs[i];
-      k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
-      launch_fill_count(c->seg, c->border, c->num_neg, c->W, c->H, s);
-      HIP_TRY(hipMemcpyAsync(c->host + 2, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, s));
+      const hipStream_t ss = lane_of(L, s, i);
+      launch_fill(c->fillcnt, 0, rounds, ss);
+      for (int r = 0; r < rounds; r++) {
+        launch_fill_count(c->seg, c->border, c->fillcnt + r, c->W, c->H, ss);
+        launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, ss);
+      }
+      HIP_TRY(hipMemcpyAsync(c->host + 4, c->fillcnt, sizeof(int) * rounds, hipMemcpyDeviceToHost,
+                             ss));
       any = 1;
     }
     if (!any) return STSP_OK;
-    HIP_TRY(hipStreamSynchronize(s));
+    int rc = lanes_sync(L, s);
+    if (rc) return rc;
     for (int i = 0; i < n; i++) {
       if (!live[i]) continue;
-      stsp_ctx* c = cs[i];
-      const int neg = c->host[2];
-      if (neg == 0) {
-        live[i] = 0;
-        continue;
+      const int* cnt = cs[i]->host + 4;
+      for (int r = 0; r < rounds; r++) {
+        const int neg = cnt[r];
+        if (neg == 0) {
+          live[i] = 0;
+          break;
+        }
+        if (base + r > 0 && neg == prev[i]) return STSP_ERR_FILL;
+        prev[i] = neg;
       }
-      if (iter > 0 && neg == prev[i]) return STSP_ERR_FILL;
-      launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, s);
-      prev[i] = neg;
     }
   }
   return STSP_OK;
@@ -753,47 +842,55 @@ static int fill_loop_n(stsp_ctx** cs, int n, hipStream_t s) {
 
 // shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up
 // (prop.cu:677-745) for frames cs[0..n), stage by stage across frames so
-// that the host waits (hole-filling rounds, invalid counts) are shared.
+// that the host waits (hole-filling rounds, invalid counts) are shared;
+// frame i runs on lane i.
 static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
                            const float* const* flows, const int* const* spix_prev,
-                           stsp_params* prevs, const int* nprev, int* niters_state,
+                           stsp_params* prevs, const int* nprev, int* niters_state, Lanes* L,
                            hipStream_t s) {
+  int rc = lanes_fork(L, s, n);
+  if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
+    const hipStream_t ss = lane_of(L, s, i);
     const int np = nprev[i];
-    int rc = ensure_labels(c, 10 * np);  // prop.cu:702
+    rc = ensure_labels(c, 10 * np);  // prop.cu:702
     if (rc) return rc;
     rc = ensure_prev(c, np);
     if (rc) return rc;
     launch_flow_pool(flows[i], spix_prev[i], c->npix, c->fsum, c->counts_prev,
-                     prevs[i].mu_shape, c->flow_ds, c->centers, np, s);
-    launch_shift(spix_prev[i], c->flow_ds, c->counts_prev, c->keys, c->shifted, c->W, c->H, s);
+                     prevs[i].mu_shape, c->flow_ds, c->centers, np, ss);
+    launch_shift(spix_prev[i], c->flow_ds, c->counts_prev, c->keys, c->shifted, c->W, c->H, ss);
     HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * c->npix, hipMemcpyDeviceToDevice,
-                           s));
-    launch_count_labels(c->shifted, c->npix, c->counts_s, np, s);
+                           ss));
+    launch_count_labels(c->shifted, c->npix, c->counts_s, np, ss);
   }
-  int rc = fill_loop_n(cs, n, s);
+  rc = fill_loop_n(cs, n, L, s);
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
-    launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, c->W, c->H, nprev[i], s);
+    launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, c->W, c->H, nprev[i],
+                                   lane_of(L, s, i));
   }
-  rc = fill_loop_n(cs, n, s);
+  rc = fill_loop_n(cs, n, L, s);
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
+    const hipStream_t ss = lane_of(L, s, i);
     const int np = nprev[i];
-    launch_count_labels(c->seg, c->npix, c->counts_f, np, s);
+    launch_count_labels(c->seg, c->npix, c->counts_f, np, ss);
     launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prevs[i].prior_counts,
-                        prevs[i].invalid_counts, np, s);
+                        prevs[i].invalid_counts, np, ss);
     // adaptive iteration count (pyapi.cu:139-149): holes of the shifted map
-    k_set_int<<<1, 1, 0, s>>>(c->num_neg, 0);
-    launch_fill_count(c->shifted, c->border, c->num_neg, c->W, c->H, s);
-    HIP_TRY(hipMemcpyAsync(c->host + 3, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, s));
+    k_set_int<<<1, 1, 0, ss>>>(c->num_neg, 0);
+    launch_fill_count(c->shifted, c->border, c->num_neg, c->W, c->H, ss);
+    HIP_TRY(hipMemcpyAsync(c->host + 3, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, ss));
   }
-  HIP_TRY(hipStreamSynchronize(s));
+  rc = lanes_sync(L, s);
+  if (rc) return rc;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
+    const hipStream_t ss = lane_of(L, s, i);
     const stsp_config& cfg = c->cfg;
     const int np = nprev[i];
     const int nbuf = 10 * np;
@@ -803,8 +900,8 @@ static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
     if (niters_state) niters_state[i] = adapt;
     InParams in;
     memcpy(&in, &prevs[i], sizeof(in));
-    launch_prop_params_init(c->P, in, np, nbuf, cfg.sp_size, c->seg, c->npix, s);
-    k_set_int<<<1, 1, 0, s>>>(c->max_spix, np - 1);
+    launch_prop_params_init(c->P, in, np, nbuf, cfg.sp_size, c->seg, c->npix, ss);
+    k_set_int<<<1, 1, 0, ss>>>(c->max_spix, np - 1);
     c->img = imgs[i];
     c->nbuf = nbuf;
     c->nspix_prev = np;
@@ -816,7 +913,7 @@ static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
     c->prev_prior_mu_app = prevs[i].prior_mu_app;
   }
   HIP_TRY(hipGetLastError());
-  return STSP_OK;
+  return lanes_join(L, s);
 }
 
 // compaction (compact_spix.cu:41-122 as called at prop.cu:820-829 and
@@ -967,7 +1064,7 @@ int stsp_bass(stsp_ctx* c, const float* img, int* spix_out, stsp_params* out, in
               hipStream_t s) {
   if (!c || !img || !spix_out || !out || !nspix_out) return STSP_ERR_ARG;
   const float* im = img;
-  int rc = bass_prologue_n(&c, &im, 1, s);
+  int rc = bass_prologue_n(&c, &im, 1, nullptr, s);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
 }
@@ -978,7 +1075,8 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
   if (!c || !img || !flow || !spix_prev || !prev || !spix_out || !out || !nspix_out ||
       nspix_prev < 1)
     return STSP_ERR_ARG;
-  int rc = prop_prologue_n(&c, 1, &img, &flow, &spix_prev, prev, &nspix_prev, niters_state, s);
+  int rc = prop_prologue_n(&c, 1, &img, &flow, &spix_prev, prev, &nspix_prev, niters_state,
+                           nullptr, s);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
 }
@@ -1035,17 +1133,22 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
     int rc = refine(bh, group.data(), (int)group.size(), s);
     if (rc) return rc;
   }
+  Lanes* L = &bh->lanes;
+  int rc = lanes_fork(L, s, n);
+  if (rc) return rc;
   for (int i = 0; i < n; i++) {
-    int rc = epilogue_compact(ctxs[i], s);
+    rc = epilogue_compact(ctxs[i], lane_of(L, s, i));
     if (rc) return rc;
   }
-  HIP_TRY(hipStreamSynchronize(s));
+  rc = lanes_sync(L, s);
+  if (rc) return rc;
+  if (L->n) HIP_TRY(hipStreamSynchronize(s));  // the refinement's timing events
   for (int i = 0; i < n; i++) tflush(ctxs[i]);  // timed launches are recorded on group leads
   for (int i = 0; i < n; i++) {
-    int rc = epilogue_finish(ctxs[i], spix_out[i], &outs[i], &nspix_out[i], s);
+    rc = epilogue_finish(ctxs[i], spix_out[i], &outs[i], &nspix_out[i], lane_of(L, s, i));
     if (rc) return rc;
   }
-  return STSP_OK;
+  return lanes_join(L, s);
 }
 
 static int check_batch_ctxs(stsp_batch* bh, stsp_ctx** ctxs, int n) {
@@ -1070,7 +1173,8 @@ int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
     return STSP_ERR_ARG;
   for (int i = 0; i < n; i++)
     if (nspix_prev[i] < 1) return STSP_ERR_ARG;
-  rc = prop_prologue_n(ctxs, n, imgs, flows, spix_prev, prevs, nspix_prev, niters_state, s);
+  rc = prop_prologue_n(ctxs, n, imgs, flows, spix_prev, prevs, nspix_prev, niters_state,
+                       &bh->lanes, s);
   if (rc) return rc;
   return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
 }
@@ -1080,7 +1184,7 @@ int stsp_bass_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs, 
   int rc = check_batch_ctxs(bh, ctxs, n);
   if (rc) return rc;
   if (!imgs || !spix_out || !outs || !nspix_out) return STSP_ERR_ARG;
-  rc = bass_prologue_n(ctxs, imgs, n, s);
+  rc = bass_prologue_n(ctxs, imgs, n, &bh->lanes, s);
   if (rc) return rc;
   return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
 }

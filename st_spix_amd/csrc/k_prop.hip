@@ -217,12 +217,19 @@ __global__ void k_cc_union(const int* __restrict__ seg, int* parent, int W, int 
   }
 }
 
+// Points every pixel at its root and counts region sizes; neighbouring
+// pixels of a wave mostly share a root, so the counts are added once per run
+// of equal roots.
 __global__ void k_cc_compress(int* parent, int* rsize, int npix) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  int r = uf_find(parent, i);
-  parent[i] = r;
-  atomicAdd(rsize + r, 1);
+  int r = -1;
+  if (i < npix) {
+    r = uf_find(parent, i);
+    parent[i] = r;
+  }
+  int run_end;
+  const bool head = wave_run(r, run_end);
+  if (head && r >= 0) atomicAdd(rsize + r, run_end - (int)(threadIdx.x & 63));
 }
 
 __global__ void k_cc_best(const int* __restrict__ seg, const int* __restrict__ root,
