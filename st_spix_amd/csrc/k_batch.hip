@@ -396,6 +396,7 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   __shared__ unsigned short list[4][LCAP];
   __shared__ int cnt[4];
   __shared__ uint32_t lutb[8];
+  __shared__ uint32_t touched[(TW * TH + 31) / 32];  // tile cells whose label moved
   const int f = frame_of(b.off.tile[SHAPE], b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
   const int H = F.H, W = F.W;
@@ -408,12 +409,29 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   const int tile = blockIdx.x - b.off.tile[SHAPE][f];
   const int tid = threadIdx.x;
   if (tid < 8) lutb[tid] = lut.bits[tid];
+  for (int i = tid; i < (TW * TH + 31) / 32; i += NT) touched[i] = 0u;
   const int tx = tile % tiles_x, ty = tile / tiles_x;
   const int x0 = tx * TW - R, y0 = ty * TH - R;
-  for (int i = tid; i < LH * LW; i += NT) {
-    int lx = i % LW, ly = i / LW;
-    int gx = x0 + lx, gy = y0 + ly;
-    lab[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? seg_in[gy * W + gx] : -1;
+  {
+    // all loads of the tile are issued before the first LDS store: each
+    // reads a clamped in-frame address, cells outside the frame get -1
+    constexpr int NL = (LH * LW + NT - 1) / NT;
+    int v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int i = tid + k * NT;
+      const int lx = i % LW, ly = i / LW;
+      const int gx = x0 + lx, gy = y0 + ly;
+      const int cx = min(max(gx, 0), W - 1), cy = min(max(gy, 0), H - 1);
+      v[k] = seg_in[cy * W + cx];
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int i = tid + k * NT;
+      const int lx = i % LW, ly = i / LW;
+      const int gx = x0 + lx, gy = y0 + ly;
+      if (i < LH * LW) lab[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? v[k] : -1;
+    }
   }
 #pragma unroll
   for (int s = 0; s < S; s++) {  // unrolled: region widths are constants
@@ -455,28 +473,73 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
         const int N = lab[li - LW], Sd = lab[li + LW], Wl = lab[li - 1], E = lab[li + 1];
         const int nb[8] = {lab[li - LW - 1], N, lab[li - LW + 1], Wl, E, lab[li + LW - 1], Sd,
                            lab[li + LW + 1]};
-        const int bN = nbits(nb, N), bW = nbits(nb, Wl), bS = nbits(nb, Sd), bE = nbits(nb, E);
-        const bool ok = ((lutb[bN >> 5] >> (bN & 31)) & 1u) && ((lutb[bW >> 5] >> (bW & 31)) & 1u) &&
-                        ((lutb[bS >> 5] >> (bS & 31)) & 1u) && ((lutb[bE >> 5] >> (bE & 31)) & 1u);
-        if (!ok) continue;
+        // The neighbour pattern of each distinct label among N, S, W, E
+        // (including -1 outside the frame) must pass the simple-point table.
+        // Candidates are those labels >= 0 in the order N, S, W, E, each
+        // once; `order` packs their indices (2 bits each).  The image value,
+        // the own label's count and the first two candidates' records are
+        // gathered before the table checks so their latency overlaps them.
+        const int q[4] = {N, Sd, Wl, E};
+        bool dup[4];
+        int order = 0, ncand = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          dup[j] = false;
+#pragma unroll
+          for (int t = 0; t < j; t++) dup[j] = dup[j] || q[t] == q[j];
+          if (!dup[j] && q[j] >= 0) {
+            order |= j << (2 * ncand);
+            ncand++;
+          }
+        }
+        const int j0 = order & 3, j1 = (order >> 2) & 3;
+        const int L0 = ncand > 0 ? (j0 == 0 ? N : j0 == 1 ? Sd : j0 == 2 ? Wl : E) : C;
+        const int L1 = ncand > 1 ? (j1 == 1 ? Sd : j1 == 2 ? Wl : E) : L0;
+        const SweepRec r0 = rec[L0], r1 = rec[L1];
+        const int cntC = rec[C].count;
         const int gidx = gy * W + gx;
         const float imgC[3] = {img[3 * gidx], img[3 * gidx + 1], img[3 * gidx + 2]};
+        int m[4];
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          m[j] = 0;
+          if (!dup[j]) {
+            m[j] = nbits(nb, q[j]);
+            ok = ok && ((lutb[m[j] >> 5] >> (m[j] & 31)) & 1u);
+          }
+        }
+        if (!ok) continue;
         float best = -9999.0f;
         int out = C;
-        if (N >= 0) joint_b(imgC, gx, gy, rec[N], N, sigma2_app, (float)(8 - __popc(bN)), potts, best, out);
-        if (Sd != N && Sd >= 0)
-          joint_b(imgC, gx, gy, rec[Sd], Sd, sigma2_app, (float)(8 - __popc(bS)), potts, best, out);
-        if (Wl != Sd && Wl != N && Wl >= 0)
-          joint_b(imgC, gx, gy, rec[Wl], Wl, sigma2_app, (float)(8 - __popc(bW)), potts, best, out);
-        if (E != Wl && E != Sd && E != N && E >= 0)
-          joint_b(imgC, gx, gy, rec[E], E, sigma2_app, (float)(8 - __popc(bE)), potts, best, out);
-        if (rec[C].count <= 2 && out == C) {
+        if (ncand > 0) {
+          const int mj = j0 == 0 ? m[0] : j0 == 1 ? m[1] : j0 == 2 ? m[2] : m[3];
+          joint_b(imgC, gx, gy, r0, L0, sigma2_app, (float)(8 - __popc(mj)), potts, best, out);
+        }
+        if (ncand > 1) {
+          const int mj = j1 == 1 ? m[1] : j1 == 2 ? m[2] : m[3];
+          joint_b(imgC, gx, gy, r1, L1, sigma2_app, (float)(8 - __popc(mj)), potts, best, out);
+        }
+        for (int c = 2; c < ncand; c++) {
+          const int j = (order >> (2 * c)) & 3;
+          const int L = j == 2 ? Wl : E;
+          const int mj = j == 2 ? m[2] : m[3];
+          joint_b(imgC, gx, gy, rec[L], L, sigma2_app, (float)(8 - __popc(mj)), potts, best, out);
+        }
+        if (cntC <= 2 && out == C) {
           if (N >= 0) out = N;
           else if (E >= 0) out = E;
           else if (Sd >= 0) out = Sd;
           else if (Wl >= 0) out = Wl;
         }
-        lab[li] = out;
+        if (out != C) {
+          lab[li] = out;
+          const int cx = lx - R, cy = ly - R;
+          if (cx >= 0 && cx < TW && cy >= 0 && cy < TH) {
+            const int k = cy * TW + cx;
+            atomicOr(&touched[k >> 5], 1u << (k & 31));
+          }
+        }
       }
       __syncthreads();
     }
@@ -492,6 +555,7 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     const int g = gy * W + gx;
     const int nl = lab[ly * LW + lx];
     seg_out[g] = nl;
+    if (!((touched[i >> 5] >> (i & 31)) & 1u)) continue;  // never moved: nl == seg_in[g]
     const int ol = seg_in[g];
     if (nl == ol) continue;
     const int64_t x = gx, y = gy;
