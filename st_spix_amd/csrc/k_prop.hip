@@ -198,22 +198,108 @@ __device__ __forceinline__ void uf_union(int* parent, int a, int b) {
   }
 }
 
-__global__ void k_cc_init(int* parent, int* rsize, int npix) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  parent[i] = i;
-  rsize[i] = 0;
+// Stage 1: one workgroup links the pixels of a kCcTile x kCcTile tile in
+// LDS (same rule: every tile component to its smallest pixel index, which is
+// also its smallest frame index because both orders are row-major), then
+// points each pixel's global parent at its tile root.  Pixels of label < 0
+// are left alone: their components never decide anything.
+constexpr int kCcTile = 32;
+
+__device__ __forceinline__ int lds_find(int* P, int i) {
+  int p = __hip_atomic_load(P + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (p != i) {
+    i = p;
+    p = __hip_atomic_load(P + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  return i;
+}
+__device__ __forceinline__ void lds_union(int* P, int a, int b) {
+  while (true) {
+    a = lds_find(P, a);
+    b = lds_find(P, b);
+    if (a == b) return;
+    if (a < b) {
+      int old = atomicMin(P + b, a);
+      if (old == b) return;
+      b = old;
+    } else {
+      int old = atomicMin(P + a, b);
+      if (old == a) return;
+      a = old;
+    }
+  }
 }
 
-__global__ void k_cc_union(const int* __restrict__ seg, int* parent, int W, int H) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W * H) return;
-  int x = i % W, y = i / W, L = seg[i];
-  if (x > 0 && seg[i - 1] == L) uf_union(parent, i, i - 1);
+__global__ void __launch_bounds__(256) k_cc_tile(const int* __restrict__ seg, int* parent,
+                                                 int* rsize, int W, int H) {
+  constexpr int T = kCcTile, N = T * T;
+  __shared__ int lab[N];
+  __shared__ int P[N];
+  const int tiles_x = (W + T - 1) / T;
+  const int tx0 = (blockIdx.x % tiles_x) * T, ty0 = (blockIdx.x / tiles_x) * T;
+  const int tid = threadIdx.x;
+  int v[N / 256];
+#pragma unroll
+  for (int k = 0; k < N / 256; k++) {
+    const int i = tid + 256 * k;
+    const int gx = tx0 + i % T, gy = ty0 + i / T;
+    v[k] = (gx < W && gy < H) ? seg[gy * W + gx] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < N / 256; k++) {
+    const int i = tid + 256 * k;
+    lab[i] = v[k];
+    P[i] = i;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N / 256; k++) {
+    const int i = tid + 256 * k;
+    const int lx = i % T, ly = i / T, L = v[k];
+    if (L < 0) continue;
+    if (lx > 0 && lab[i - 1] == L) lds_union(P, i, i - 1);
+    if (ly > 0) {
+      if (lab[i - T] == L) lds_union(P, i, i - T);
+      if (lx > 0 && lab[i - T - 1] == L) lds_union(P, i, i - T - 1);
+      if (lx < T - 1 && lab[i - T + 1] == L) lds_union(P, i, i - T + 1);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N / 256; k++) {
+    const int i = tid + 256 * k;
+    const int gx = tx0 + i % T, gy = ty0 + i / T;
+    if (gx >= W || gy >= H) continue;
+    const int r = lds_find(P, i);
+    parent[gy * W + gx] = (ty0 + r / T) * W + tx0 + r % T;
+    rsize[gy * W + gx] = 0;
+  }
+}
+
+// Stage 2: the neighbour pairs that cross a tile edge (each pixel's W, N, NW
+// and NE neighbours; a pair is owned by its later pixel) are linked in the
+// global forest, whose chains now run only between tile roots.
+__global__ void __launch_bounds__(128) k_cc_seams(const int* __restrict__ seg, int* parent, int W,
+                                                  int H) {
+  constexpr int T = kCcTile;
+  const int tiles_x = (W + T - 1) / T;
+  const int tx0 = (blockIdx.x % tiles_x) * T, ty0 = (blockIdx.x / tiles_x) * T;
+  const int t = threadIdx.x;
+  if (t >= 3 * T) return;
+  // t < T: top row; T <= t < 2T: left column; 2T <= t: right column
+  const int lx = t < T ? t : (t < 2 * T ? 0 : T - 1);
+  const int ly = t < T ? 0 : t % T;
+  const int x = tx0 + lx, y = ty0 + ly;
+  if (x >= W || y >= H) return;
+  const int i = y * W + x, L = seg[i];
+  if (L < 0) return;
+  const bool top = ly == 0, left = lx == 0, right = lx == T - 1;
+  if (t >= T && top) return;  // the top row owns its corners
+  if (left && x > 0 && seg[i - 1] == L) uf_union(parent, i, i - 1);
   if (y > 0) {
-    if (seg[i - W] == L) uf_union(parent, i, i - W);
-    if (x > 0 && seg[i - W - 1] == L) uf_union(parent, i, i - W - 1);
-    if (x < W - 1 && seg[i - W + 1] == L) uf_union(parent, i, i - W + 1);
+    if (top && seg[i - W] == L) uf_union(parent, i, i - W);
+    if ((top || left) && x > 0 && seg[i - W - 1] == L) uf_union(parent, i, i - W - 1);
+    if ((top || right) && x < W - 1 && seg[i - W + 1] == L) uf_union(parent, i, i - W + 1);
   }
 }
 
@@ -257,9 +343,10 @@ __global__ void k_cc_apply(int* seg, const int* __restrict__ root,
 void launch_invalidate_disconnected(int* seg, int* parent, int* rsize, unsigned long long* best,
                                     int W, int H, int n, hipStream_t s) {
   int npix = W * H;
-  k_cc_init<<<div_up(npix, kBlock), kBlock, 0, s>>>(parent, rsize, npix);
+  const int tiles = div_up(W, kCcTile) * div_up(H, kCcTile);
   k_u64_fill<<<div_up(n, kBlock), kBlock, 0, s>>>(best, 0ull, n);
-  k_cc_union<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, parent, W, H);
+  k_cc_tile<<<tiles, 256, 0, s>>>(seg, parent, rsize, W, H);
+  k_cc_seams<<<tiles, 128, 0, s>>>(seg, parent, W, H);
   k_cc_compress<<<div_up(npix, kBlock), kBlock, 0, s>>>(parent, rsize, npix);
   k_cc_best<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, parent, rsize, best, npix, n);
   k_cc_apply<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, parent, best, npix, n);
