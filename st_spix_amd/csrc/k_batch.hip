@@ -774,17 +774,21 @@ __device__ __forceinline__ void bfs_wave(const int* __restrict__ seg, int W, int
   uint64_t M[RC][WD], V1[RC][WD], V2[RC][WD], NK[RC][WD];
 #pragma unroll
   for (int c = 0; c < RC; c++) {
+    const int nrows = min(64, bh - 64 * c);  // wave-uniform
 #pragma unroll
     for (int w = 0; w < WD; w++) {
       M[c][w] = V1[c][w] = V2[c][w] = NK[c][w] = 0;
       const int col = 64 * w + lane;
-#pragma unroll 8
-      for (int rr = 0; rr < 64; rr++) {
-        const int row = 64 * c + rr;
-        bool bit = false;
-        if (row < bh && col < bw) bit = seg[(by0 + row) * W + bx0 + col] == k;
-        const uint64_t word = __ballot(bit);
-        M[c][w] = lane == rr ? word : M[c][w];
+      for (int r0 = 0; r0 < nrows; r0 += 16) {
+        int v[16];  // 16 row loads in flight, then 16 ballots
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+          v[j] = (r0 + j < nrows && col < bw) ? seg[(by0 + 64 * c + r0 + j) * W + bx0 + col] : -1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const uint64_t word = __ballot(v[j] == k);
+          M[c][w] = lane == r0 + j ? word : M[c][w];
+        }
       }
     }
   }
@@ -822,12 +826,10 @@ __device__ __forceinline__ void bfs_wave(const int* __restrict__ seg, int W, int
     for (int w = 0; w < WD; w++) {
       const uint64_t K = V1[c][w] & ~NK[c][w];
       const int col = 64 * w + lane;
-#pragma unroll 8
-      for (int rr = 0; rr < 64; rr++) {
+      const int nrows = min(64, bh - 64 * c);
+      for (int rr = 0; rr < nrows; rr++) {
         const uint64_t word = __shfl(K, rr);
-        const int row = 64 * c + rr;
-        if (row < bh && col < bw && ((word >> lane) & 1ull))
-          halves[(by0 + row) * W + bx0 + col] = k2;
+        if (col < bw && ((word >> lane) & 1ull)) halves[(by0 + 64 * c + rr) * W + bx0 + col] = k2;
       }
     }
   }
@@ -855,6 +857,7 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
     const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
     const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
     const int W = F.W;
+    if (s1 < 0) continue;  // no pixel has d1 > d2: the label keeps all its pixels
     if (bh <= 64) {
       if (bw <= 64) bfs_wave<1, 1>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
       else bfs_wave<1, 2>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves);
@@ -865,14 +868,158 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
   }
 }
 
-// Persistent workgroups over the block list: bitsets in LDS (up to
-// kBlkWords words), level-synchronous with two barriers per level; boxes
-// beyond that use per-pixel distances in global memory.
-constexpr int kBlkWords = 2048;
+// Boxes beyond the wave path: one workgroup per label, thread t holding rows
+// t + 256c (c < RC) of the box, WD words each, in registers.  Rows cross
+// waves (and chunks) through LDS: each wave publishes the first and last row
+// of every chunk per level (double-buffered by level parity), and a
+// triple-buffered flag records growth, so one barrier ends a level.  Up to
+// 256 * kRowsChunks rows and 64 * kRowsWords columns; larger boxes use LDS
+// bitsets (up to kBlkWords words) or per-pixel distances in global memory.
+constexpr int kRowsChunks = 4, kRowsWords = 6, kBlkWords = 2048;
+
+struct RowsEdge {
+  uint64_t v[2][kRowsChunks][4][2][2][kRowsWords];  // [buf][chunk][wave][first|last][seed][word]
+};
+
+template <int RC, int WD>
+__device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int k, int bx0,
+                                         int by0, int bw, int bh, int s1, int s2, int k2,
+                                         int* __restrict__ halves, RowsEdge& E, int* grow) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint64_t M[RC][WD], V1[RC][WD], V2[RC][WD], NK[RC][WD];
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+    const int nrows = min(64, bh - 256 * c - 64 * wv);  // wave-uniform, may be <= 0
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      M[c][w] = V1[c][w] = V2[c][w] = NK[c][w] = 0;
+      const int col = 64 * w + lane;
+      for (int r0 = 0; r0 < nrows; r0 += 16) {
+        int v[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+          v[j] = (r0 + j < nrows && col < bw)
+                     ? seg[(by0 + 256 * c + 64 * wv + r0 + j) * W + bx0 + col]
+                     : -1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const uint64_t word = __ballot(v[j] == k);
+          M[c][w] = lane == r0 + j ? word : M[c][w];
+        }
+      }
+    }
+  }
+  {
+    const int y1 = s1 / W - by0, x1 = s1 % W - bx0;
+    const int y2 = s2 >= 0 ? s2 / W - by0 : -1, x2 = s2 >= 0 ? s2 % W - bx0 : 0;
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        if (y1 == 256 * c + tid && (x1 >> 6) == w) V1[c][w] |= 1ull << (x1 & 63);
+        if (y2 == 256 * c + tid && (x2 >> 6) == w) V2[c][w] |= 1ull << (x2 & 63);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      if (lane == 0) {
+        E.v[0][c][wv][0][0][w] = V1[c][w];
+        E.v[0][c][wv][0][1][w] = V2[c][w];
+      }
+      if (lane == 63) {
+        E.v[0][c][wv][1][0][w] = V1[c][w];
+        E.v[0][c][wv][1][1][w] = V2[c][w];
+      }
+    }
+  }
+  if (tid < 3) grow[tid] = 0;
+  __syncthreads();
+  for (int level = 0;; level++) {
+    const int rb = level & 1, wb = rb ^ 1;
+    uint64_t N1[RC][WD], N2[RC][WD];
+    uint64_t grew = 0;
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+      // the row above lane 0 / below lane 63: the neighbouring wave of this
+      // chunk, or the last (first) wave of the previous (next) chunk
+      const int uc = wv > 0 ? c : c - 1, uw = wv > 0 ? wv - 1 : 3;
+      const int dc = wv < 3 ? c : c + 1, dw = wv < 3 ? wv + 1 : 0;
+      const bool has_u = uc >= 0, has_d = dc < RC;
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        uint64_t h1 = (V1[c][w] << 1) | (V1[c][w] >> 1), h2 = (V2[c][w] << 1) | (V2[c][w] >> 1);
+        if (w > 0) {
+          h1 |= V1[c][w - 1] >> 63;
+          h2 |= V2[c][w - 1] >> 63;
+        }
+        if (w + 1 < WD) {
+          h1 |= V1[c][w + 1] << 63;
+          h2 |= V2[c][w + 1] << 63;
+        }
+        uint64_t u1 = __shfl_up(V1[c][w], 1), u2 = __shfl_up(V2[c][w], 1);
+        uint64_t d1 = __shfl_down(V1[c][w], 1), d2 = __shfl_down(V2[c][w], 1);
+        const uint64_t eu1 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][0][w] : 0ull;
+        const uint64_t eu2 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][1][w] : 0ull;
+        const uint64_t ed1 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][0][w] : 0ull;
+        const uint64_t ed2 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][1][w] : 0ull;
+        u1 = lane == 0 ? eu1 : u1;
+        u2 = lane == 0 ? eu2 : u2;
+        d1 = lane == 63 ? ed1 : d1;
+        d2 = lane == 63 ? ed2 : d2;
+        N1[c][w] = (h1 | u1 | d1) & M[c][w] & ~V1[c][w];
+        N2[c][w] = (h2 | u2 | d2) & M[c][w] & ~V2[c][w];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        V1[c][w] |= N1[c][w];
+        NK[c][w] |= N2[c][w] & V1[c][w];
+        V2[c][w] |= N2[c][w];
+        grew |= N1[c][w] | N2[c][w];
+        if (lane == 0) {
+          E.v[wb][c][wv][0][0][w] = V1[c][w];
+          E.v[wb][c][wv][0][1][w] = V2[c][w];
+        }
+        if (lane == 63) {
+          E.v[wb][c][wv][1][0][w] = V1[c][w];
+          E.v[wb][c][wv][1][1][w] = V2[c][w];
+        }
+      }
+    }
+    // grow[level % 3] is cleared two levels before it is set, after every
+    // wave has read it for the last time (that read precedes the barrier
+    // that ends the level before the clear)
+    const bool any = __ballot(grew != 0) != 0;
+    if (any && lane == 0) grow[level % 3] = 1;
+    if (tid == 0) grow[(level + 1) % 3] = 0;
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(grow[level % 3]) == 0) break;
+  }
+#pragma unroll
+  for (int c = 0; c < RC; c++) {
+    const int nrows = min(64, bh - 256 * c - 64 * wv);
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      const uint64_t K = V1[c][w] & ~NK[c][w];
+      const int col = 64 * w + lane;
+      for (int rr = 0; rr < nrows; rr++) {
+        const uint64_t word = __shfl(K, rr);
+        if (col < bw && ((word >> lane) & 1ull))
+          halves[(by0 + 256 * c + 64 * wv + rr) * W + bx0 + col] = k2;
+      }
+    }
+  }
+}
 
 __global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direction) {
   __shared__ uint64_t sM[kBlkWords], sV1[kBlkWords], sV2[kBlkWords], sNK[kBlkWords];
-  __shared__ int item, changed[2];
+  __shared__ RowsEdge edge;
+  __shared__ int item, changed[3];
   const int tid = threadIdx.x;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
   while (true) {
@@ -895,6 +1042,28 @@ __global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direct
     const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
     const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
     int* halves = F.halves;
+    if (s1 < 0) continue;  // no pixel has d1 > d2: the label keeps all its pixels
+#define STSP_ROWS(RC, WD) \
+  bfs_rows<RC, WD>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, edge, changed)
+    // (4 chunks of 6 words would not fit in registers)
+    if (bh <= 256 * kRowsChunks && bw <= 64 * kRowsWords && (bh <= 512 || bw <= 256)) {
+      const int rc = bh <= 256 ? 1 : (bh <= 512 ? 2 : 4);
+      const int wd = bw <= 128 ? 2 : (bw <= 256 ? 4 : 6);
+      if (rc == 1) {
+        if (wd == 2) STSP_ROWS(1, 2);
+        else if (wd == 4) STSP_ROWS(1, 4);
+        else STSP_ROWS(1, 6);
+      } else if (rc == 2) {
+        if (wd == 2) STSP_ROWS(2, 2);
+        else if (wd == 4) STSP_ROWS(2, 4);
+        else STSP_ROWS(2, 6);
+      } else {
+        if (wd == 2) STSP_ROWS(4, 2);
+        else STSP_ROWS(4, 4);
+      }
+      continue;
+    }
+#undef STSP_ROWS
     const int WD = (bw + 63) >> 6;
     const int nw = bh * WD;
     if (nw <= kBlkWords) {
