@@ -182,6 +182,29 @@ __device__ __forceinline__ int64_t run_sum64(int64_t v, int run_end) {
   return v;
 }
 
+// lane i takes lane i-1's value, lane 0 takes `first` (DPP wave_shr:1)
+__device__ __forceinline__ uint64_t lane_up(uint64_t v, uint64_t first) {
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)first, (int)(uint32_t)v, 0x138, 0xf,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(first >> 32), (int)(uint32_t)(v >> 32),
+                                             0x138, 0xf, 0xf, false);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+// lane i takes lane i+1's value, lane 63 takes `last` (DPP wave_shl:1)
+__device__ __forceinline__ uint64_t lane_down(uint64_t v, uint64_t last) {
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)last, (int)(uint32_t)v, 0x130, 0xf,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(last >> 32), (int)(uint32_t)(v >> 32),
+                                             0x130, 0xf, 0xf, false);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+// a 64-bit value of one lane, broadcast
+__device__ __forceinline__ uint64_t lane_get(uint64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace stsp

@@ -753,12 +753,9 @@ __device__ __forceinline__ void bits_expand(const uint64_t (&V)[RC][WD],
       uint64_t h = (v << 1) | (v >> 1);
       if (w > 0) h |= V[c][w - 1] >> 63;
       if (w + 1 < WD) h |= V[c][w + 1] << 63;
-      uint64_t up = __shfl_up(v, 1);
-      const uint64_t last_prev = c > 0 ? __shfl(V[c > 0 ? c - 1 : 0][w], 63) : 0ull;
-      up = lane == 0 ? last_prev : up;
-      uint64_t dn = __shfl_down(v, 1);
-      const uint64_t first_next = c + 1 < RC ? __shfl(V[c + 1 < RC ? c + 1 : c][w], 0) : 0ull;
-      dn = lane == 63 ? first_next : dn;
+      const uint64_t last_prev = c > 0 ? lane_get(V[c > 0 ? c - 1 : 0][w], 63) : 0ull;
+      const uint64_t first_next = c + 1 < RC ? lane_get(V[c + 1 < RC ? c + 1 : c][w], 0) : 0ull;
+      const uint64_t up = lane_up(v, last_prev), dn = lane_down(v, first_next);
       N[c][w] = (h | up | dn) & M[c][w] & ~v;
     }
   }
@@ -959,16 +956,12 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
           h1 |= V1[c][w + 1] << 63;
           h2 |= V2[c][w + 1] << 63;
         }
-        uint64_t u1 = __shfl_up(V1[c][w], 1), u2 = __shfl_up(V2[c][w], 1);
-        uint64_t d1 = __shfl_down(V1[c][w], 1), d2 = __shfl_down(V2[c][w], 1);
         const uint64_t eu1 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][0][w] : 0ull;
         const uint64_t eu2 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][1][w] : 0ull;
         const uint64_t ed1 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][0][w] : 0ull;
         const uint64_t ed2 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][1][w] : 0ull;
-        u1 = lane == 0 ? eu1 : u1;
-        u2 = lane == 0 ? eu2 : u2;
-        d1 = lane == 63 ? ed1 : d1;
-        d2 = lane == 63 ? ed2 : d2;
+        const uint64_t u1 = lane_up(V1[c][w], eu1), u2 = lane_up(V2[c][w], eu2);
+        const uint64_t d1 = lane_down(V1[c][w], ed1), d2 = lane_down(V2[c][w], ed2);
         N1[c][w] = (h1 | u1 | d1) & M[c][w] & ~V1[c][w];
         N2[c][w] = (h2 | u2 | d2) & M[c][w] & ~V2[c][w];
       }
@@ -981,13 +974,16 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
         NK[c][w] |= N2[c][w] & V1[c][w];
         V2[c][w] |= N2[c][w];
         grew |= N1[c][w] | N2[c][w];
-        if (lane == 0) {
-          E.v[wb][c][wv][0][0][w] = V1[c][w];
-          E.v[wb][c][wv][0][1][w] = V2[c][w];
-        }
-        if (lane == 63) {
-          E.v[wb][c][wv][1][0][w] = V1[c][w];
-          E.v[wb][c][wv][1][1][w] = V2[c][w];
+      }
+    }
+    if (lane == 0 || lane == 63) {
+      const int side = lane == 63;
+#pragma unroll
+      for (int c = 0; c < RC; c++) {
+#pragma unroll
+        for (int w = 0; w < WD; w++) {
+          E.v[wb][c][wv][side][0][w] = V1[c][w];
+          E.v[wb][c][wv][side][1][w] = V2[c][w];
         }
       }
     }
