@@ -94,7 +94,13 @@ void b_param_summary(const Batch& b, int cur, hipStream_t s);
 void b_store_sample(const Batch& b, hipStream_t s);
 int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut& lut,
              hipStream_t s);
-void b_split(const Batch& b, int cur, int direction, hipStream_t s);
+// A second stream with fork/join events: launches issued on it overlap the
+// main stream's between the fork and the join (also inside a graph capture).
+struct SideStream {
+  hipStream_t st;
+  hipEvent_t fork, join;
+};
+void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side);
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s);
 void b_relabel(const Batch& b, int cur, hipStream_t s);
 void b_count_living(const Batch& b, int slot, hipStream_t s);

@@ -110,6 +110,7 @@ struct Lanes {
 
 // A batch table on the device plus the cached refinement graph.
 struct stsp_batch {
+  SideStream side = {nullptr, nullptr, nullptr};
   int maxf = 0;
   FrameDesc* d_fr = nullptr;
   int* d_off = nullptr;  // kOffArrays arrays of maxf+1
@@ -378,6 +379,13 @@ static void lanes_free(Lanes* L) {
 
 static void batch_free(stsp_batch* bh) {
   lanes_free(&bh->lanes);
+  if (bh->side.st) {
+    (void)hipStreamSynchronize(bh->side.st);
+    (void)hipStreamDestroy(bh->side.st);
+    (void)hipEventDestroy(bh->side.fork);
+    (void)hipEventDestroy(bh->side.join);
+    bh->side = {nullptr, nullptr, nullptr};
+  }
   invalidate_graph(bh);
   if (bh->d_fr) (void)hipFree(bh->d_fr);
   if (bh->d_off) (void)hipFree(bh->d_off);
@@ -493,6 +501,11 @@ static FrameDesc make_desc(stsp_ctx* c) {
 // frame's capacity so that a captured graph stays valid across frames.
 static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   if (n > bh->maxf || n > 64) return STSP_ERR_ARG;
+  if (!bh->side.st) {
+    HIP_TRY(hipStreamCreateWithFlags(&bh->side.st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&bh->side.fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&bh->side.join, hipEventDisableTiming));
+  }
   const int M = bh->maxf + 1;
   int* off = bh->h_off;
   int* o_pix = off;
@@ -559,12 +572,12 @@ static inline bool is_merge_iter(const stsp_config& cfg, int idx) {
 // (prop.cu:364-457) for every frame of the batch; the update of the
 // parameters at its start is done by the caller.  `lead` supplies the
 // settings shared by the batch and the timers.
-static void iteration_tail(stsp_ctx* lead, const Batch& b, int idx, int prop_mode, int shape,
-                           int* cur, int* smc, hipStream_t s) {
+static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& side, int idx,
+                           int prop_mode, int shape, int* cur, int* smc, hipStream_t s) {
   const stsp_config& cfg = lead->cfg;
   if (is_split_iter(cfg, idx)) {
     *smc += 1;
-    b_split(b, *cur, *smc % 2 + 1, s);
+    b_split(b, *cur, *smc % 2 + 1, s, side);
     b_update_params(b, *cur, false, s);
   }
   if (is_merge_iter(cfg, idx)) {
@@ -595,8 +608,8 @@ static void update_at_start(const Batch& b, const stsp_config& cfg, int idx, int
 
 // The refinement loop for every frame of the batch; returns the live
 // buffer index.
-static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_mode,
-                           int shape, hipStream_t s) {
+static int refine_launches(stsp_ctx* lead, const Batch& b, const SideStream& side, int niters,
+                           int prop_mode, int shape, hipStream_t s) {
   const stsp_config& cfg = lead->cfg;
   int cur = 0;
   int smc = 1;  // split/merge counter (bass.cu:56, prop.cu:343)
@@ -604,7 +617,7 @@ static int refine_launches(stsp_ctx* lead, const Batch& b, int niters, int prop_
     tstart(lead, STSP_KCLASS_STATS, s);
     update_at_start(b, cfg, idx, cur, s);
     tstop(lead, STSP_KCLASS_STATS, s);
-    iteration_tail(lead, b, idx, prop_mode, shape, &cur, &smc, s);
+    iteration_tail(lead, b, side, idx, prop_mode, shape, &cur, &smc, s);
   }
   b_update_params(b, cur, false, s);
   b_store_sample(b, s);
@@ -666,7 +679,7 @@ static int refine_controlled(stsp_batch* bh, stsp_ctx* c, hipStream_t s) {
       rc = batch_build(bh, &one, 1, s);  // the stream is idle here: the table can be rewritten
       if (rc) return rc;
     }
-    iteration_tail(c, bh->b, idx, c->prop_mode, c->sweep_shape, &cur, &smc, s);
+    iteration_tail(c, bh->b, bh->side, idx, c->prop_mode, c->sweep_shape, &cur, &smc, s);
   }
   b_update_params(bh->b, cur, false, s);
   b_store_sample(bh->b, s);
@@ -707,7 +720,7 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     if (!bh->gexec || key != bh->gkey) {
       invalidate_graph(bh);
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int c0 = refine_launches(lead, bh->b, niters, prop_mode, shape, s);
+      int c0 = refine_launches(lead, bh->b, bh->side, niters, prop_mode, shape, s);
       hipGraph_t g = nullptr;
       const hipError_t le = hipGetLastError();
       const hipError_t ee = hipStreamEndCapture(s, &g);  // always leave capture mode
@@ -727,7 +740,7 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     HIP_TRY(hipGraphLaunch(bh->gexec, s));
     cur = bh->gcur;
   } else {
-    cur = refine_launches(lead, bh->b, niters, prop_mode, shape, s);
+    cur = refine_launches(lead, bh->b, bh->side, niters, prop_mode, shape, s);
   }
   for (int i = 0; i < n; i++) {
     if (cur == 1) {

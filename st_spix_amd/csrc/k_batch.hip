@@ -868,20 +868,23 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
 // Boxes beyond the wave path: one workgroup per label, thread t holding rows
 // t + 256c (c < RC) of the box, WD words each, in registers.  Rows cross
 // waves (and chunks) through LDS: each wave publishes the first and last row
-// of every chunk per level (double-buffered by level parity), and a
-// triple-buffered flag records growth, so one barrier ends a level.  Up to
+// of every chunk per level, and which of its units (chunk, word, seed) grew,
+// all double-buffered by level parity, so one barrier ends a level.  A unit
+// is expanded only if it or an adjacent unit grew in the level before:
+// otherwise nothing it reads has changed and its new front is empty.  Up to
 // 256 * kRowsChunks rows and 64 * kRowsWords columns; larger boxes use LDS
 // bitsets (up to kBlkWords words) or per-pixel distances in global memory.
 constexpr int kRowsChunks = 4, kRowsWords = 6, kBlkWords = 2048;
 
 struct RowsEdge {
   uint64_t v[2][kRowsChunks][4][2][2][kRowsWords];  // [buf][chunk][wave][first|last][seed][word]
+  uint32_t g[2][4];  // [buf][wave]: bit 2 (c WD + w) + seed set if that unit grew
 };
 
 template <int RC, int WD>
 __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int k, int bx0,
                                          int by0, int bw, int bh, int s1, int s2, int k2,
-                                         int* __restrict__ halves, RowsEdge& E, int* grow) {
+                                         int* __restrict__ halves, RowsEdge& E) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   uint64_t M[RC][WD], V1[RC][WD], V2[RC][WD], NK[RC][WD];
 #pragma unroll
@@ -932,12 +935,30 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
       }
     }
   }
-  if (tid < 3) grow[tid] = 0;
+  if (tid < 4) E.g[0][tid] = 0xffffffffu;  // level 0 expands every unit
   __syncthreads();
+  // unit bits: (c WD + w) 2 + seed; a unit's horizontal neighbours are the
+  // same chunk's words w - 1 and w + 1
+  uint32_t not_first_w = 0, not_last_w = 0;
+#pragma unroll
+  for (int c = 0; c < RC; c++)
+#pragma unroll
+    for (int w = 0; w < WD; w++) {
+      if (w > 0) not_first_w |= 3u << (2 * (c * WD + w));
+      if (w + 1 < WD) not_last_w |= 3u << (2 * (c * WD + w));
+    }
   for (int level = 0;; level++) {
     const int rb = level & 1, wb = rb ^ 1;
+    uint32_t active;
+    {
+      const uint32_t gm = __builtin_amdgcn_readfirstlane(E.g[rb][wv]);
+      const uint32_t gu = __builtin_amdgcn_readfirstlane(
+          wv > 0 ? E.g[rb][wv > 0 ? wv - 1 : 0] : (RC > 1 ? E.g[rb][3] << (2 * WD) : 0u));
+      const uint32_t gd = __builtin_amdgcn_readfirstlane(
+          wv < 3 ? E.g[rb][wv < 3 ? wv + 1 : 3] : (RC > 1 ? E.g[rb][0] >> (2 * WD) : 0u));
+      active = gm | ((gm << 2) & not_first_w) | ((gm >> 2) & not_last_w) | gu | gd;
+    }
     uint64_t N1[RC][WD], N2[RC][WD];
-    uint64_t grew = 0;
 #pragma unroll
     for (int c = 0; c < RC; c++) {
       // the row above lane 0 / below lane 63: the neighbouring wave of this
@@ -947,25 +968,27 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
       const bool has_u = uc >= 0, has_d = dc < RC;
 #pragma unroll
       for (int w = 0; w < WD; w++) {
-        uint64_t h1 = (V1[c][w] << 1) | (V1[c][w] >> 1), h2 = (V2[c][w] << 1) | (V2[c][w] >> 1);
-        if (w > 0) {
-          h1 |= V1[c][w - 1] >> 63;
-          h2 |= V2[c][w - 1] >> 63;
+        const int bit = 2 * (c * WD + w);
+        N1[c][w] = N2[c][w] = 0;
+        if ((active >> bit) & 1u) {
+          const uint64_t eu = has_u ? E.v[rb][has_u ? uc : 0][uw][1][0][w] : 0ull;
+          const uint64_t ed = has_d ? E.v[rb][has_d ? dc : 0][dw][0][0][w] : 0ull;
+          uint64_t h = (V1[c][w] << 1) | (V1[c][w] >> 1);
+          if (w > 0) h |= V1[c][w - 1] >> 63;
+          if (w + 1 < WD) h |= V1[c][w + 1] << 63;
+          N1[c][w] = (h | lane_up(V1[c][w], eu) | lane_down(V1[c][w], ed)) & M[c][w] & ~V1[c][w];
         }
-        if (w + 1 < WD) {
-          h1 |= V1[c][w + 1] << 63;
-          h2 |= V2[c][w + 1] << 63;
+        if ((active >> (bit + 1)) & 1u) {
+          const uint64_t eu = has_u ? E.v[rb][has_u ? uc : 0][uw][1][1][w] : 0ull;
+          const uint64_t ed = has_d ? E.v[rb][has_d ? dc : 0][dw][0][1][w] : 0ull;
+          uint64_t h = (V2[c][w] << 1) | (V2[c][w] >> 1);
+          if (w > 0) h |= V2[c][w - 1] >> 63;
+          if (w + 1 < WD) h |= V2[c][w + 1] << 63;
+          N2[c][w] = (h | lane_up(V2[c][w], eu) | lane_down(V2[c][w], ed)) & M[c][w] & ~V2[c][w];
         }
-        const uint64_t eu1 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][0][w] : 0ull;
-        const uint64_t eu2 = has_u ? E.v[rb][has_u ? uc : 0][uw][1][1][w] : 0ull;
-        const uint64_t ed1 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][0][w] : 0ull;
-        const uint64_t ed2 = has_d ? E.v[rb][has_d ? dc : 0][dw][0][1][w] : 0ull;
-        const uint64_t u1 = lane_up(V1[c][w], eu1), u2 = lane_up(V2[c][w], eu2);
-        const uint64_t d1 = lane_down(V1[c][w], ed1), d2 = lane_down(V2[c][w], ed2);
-        N1[c][w] = (h1 | u1 | d1) & M[c][w] & ~V1[c][w];
-        N2[c][w] = (h2 | u2 | d2) & M[c][w] & ~V2[c][w];
       }
     }
+    uint32_t grew = 0;
 #pragma unroll
     for (int c = 0; c < RC; c++) {
 #pragma unroll
@@ -973,7 +996,9 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
         V1[c][w] |= N1[c][w];
         NK[c][w] |= N2[c][w] & V1[c][w];
         V2[c][w] |= N2[c][w];
-        grew |= N1[c][w] | N2[c][w];
+        const int bit = 2 * (c * WD + w);
+        if (__ballot(N1[c][w] != 0) != 0) grew |= 1u << bit;
+        if (__ballot(N2[c][w] != 0) != 0) grew |= 2u << bit;
       }
     }
     if (lane == 0 || lane == 63) {
@@ -987,14 +1012,10 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
         }
       }
     }
-    // grow[level % 3] is cleared two levels before it is set, after every
-    // wave has read it for the last time (that read precedes the barrier
-    // that ends the level before the clear)
-    const bool any = __ballot(grew != 0) != 0;
-    if (any && lane == 0) grow[level % 3] = 1;
-    if (tid == 0) grow[(level + 1) % 3] = 0;
+    if (lane == 0) E.g[wb][wv] = grew;
     __syncthreads();
-    if (__builtin_amdgcn_readfirstlane(grow[level % 3]) == 0) break;
+    const uint32_t any = E.g[wb][0] | E.g[wb][1] | E.g[wb][2] | E.g[wb][3];
+    if (__builtin_amdgcn_readfirstlane(any) == 0) break;
   }
 #pragma unroll
   for (int c = 0; c < RC; c++) {
@@ -1040,7 +1061,7 @@ __global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direct
     int* halves = F.halves;
     if (s1 < 0) continue;  // no pixel has d1 > d2: the label keeps all its pixels
 #define STSP_ROWS(RC, WD) \
-  bfs_rows<RC, WD>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, edge, changed)
+  bfs_rows<RC, WD>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, edge)
     // (4 chunks of 6 words would not fit in registers)
     if (bh <= 256 * kRowsChunks && bw <= 64 * kRowsWords && (bh <= 512 || bw <= 256)) {
       const int rc = bh <= 256 ? 1 : (bh <= 512 ? 2 : 4);
@@ -1295,13 +1316,19 @@ __global__ void kb_add_max(Batch b) {
   *F.max_spix = ms + min(*F.total, F.nbuf - 1 - ms);
 }
 
-void b_split(const Batch& b, int cur, int direction, hipStream_t s) {
+// The two BFS kernels write disjoint pixels (different labels): the few
+// long workgroup searches run on the side stream beside the wave searches.
+void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_copy_halves<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_reset_work<<<1, 64, 0, s>>>(b);
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  (void)hipEventRecord(side.fork, s);
+  (void)hipStreamWaitEvent(side.st, side.fork, 0);
+  kb_bfs_block<<<256, 256, 0, side.st>>>(b, cur, direction);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
-  kb_bfs_block<<<256, 256, 0, s>>>(b, cur, direction);
+  (void)hipEventRecord(side.join, side.st);
+  (void)hipStreamWaitEvent(s, side.join, 0);
   kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
