@@ -100,6 +100,7 @@ struct KTimer {
 // small launches of different frames overlap.  All streams have the default
 // priority.  With lanes == 0 everything runs on the caller's stream.
 constexpr int kLanes = 4;
+constexpr int kFillChunk = 8;  // hole-filling rounds between two host checks
 struct Lanes {
   hipStream_t st[kLanes] = {};
   hipEvent_t fork = nullptr;
@@ -125,6 +126,16 @@ struct stsp_batch {
   std::vector<long long> gkey;
   int gcur = 0;
   Lanes lanes;
+  // propagation prologue: per-frame table, block offsets, per-round hole
+  // counts ([kFillMax][nf] on the device, one chunk of rounds on the host)
+  ProFrame* d_pro = nullptr;
+  int* d_pro_off = nullptr;
+  int* d_fillcnt = nullptr;
+  int* d_negs = nullptr;
+  ProFrame* h_pro = nullptr;
+  int* h_pro_off = nullptr;
+  int* h_fillcnt = nullptr;
+  int* h_negs = nullptr;
 };
 
 struct stsp_ctx {
@@ -153,8 +164,7 @@ struct stsp_ctx {
   double* centers;
   unsigned long long* best;
   // scalars
-  int *max_spix, *total, *num_new, *num_neg, *counter;
-  int* fillcnt;  // hole counts of a chunk of filling rounds (kFillChunk)
+  int *max_spix, *total, *num_new, *counter;
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
@@ -191,9 +201,7 @@ static int alloc_pixels(stsp_ctx* c) {
   c->max_spix = a.take<int>(1);
   c->total = a.take<int>(1);
   c->num_new = a.take<int>(1);
-  c->num_neg = a.take<int>(1);
   c->counter = a.take<int>(4);
-  c->fillcnt = a.take<int>(16);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
 }
@@ -321,6 +329,16 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipMalloc((void**)&bh->d_work_n, sizeof(int) * 4));
   HIP_TRY(hipHostMalloc((void**)&bh->h_fr, sizeof(FrameDesc) * maxf, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_off, sizeof(int) * kOffArrays * (maxf + 1), hipHostMallocDefault));
+  HIP_TRY(hipMalloc((void**)&bh->d_pro, sizeof(ProFrame) * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
+  HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_pro, sizeof(ProFrame) * maxf, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_pro_off, sizeof(int) * kProKinds * (kProMax + 1),
+                        hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_fillcnt, sizeof(int) * kFillChunk * maxf,
+                        hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf, hipHostMallocDefault));
   return STSP_OK;
 }
 
@@ -393,6 +411,20 @@ static void batch_free(stsp_batch* bh) {
   if (bh->d_work_n) (void)hipFree(bh->d_work_n);
   if (bh->h_fr) (void)hipHostFree(bh->h_fr);
   if (bh->h_off) (void)hipHostFree(bh->h_off);
+  if (bh->d_pro) (void)hipFree(bh->d_pro);
+  if (bh->d_pro_off) (void)hipFree(bh->d_pro_off);
+  if (bh->d_fillcnt) (void)hipFree(bh->d_fillcnt);
+  if (bh->d_negs) (void)hipFree(bh->d_negs);
+  if (bh->h_pro) (void)hipHostFree(bh->h_pro);
+  if (bh->h_pro_off) (void)hipHostFree(bh->h_pro_off);
+  if (bh->h_fillcnt) (void)hipHostFree(bh->h_fillcnt);
+  if (bh->h_negs) (void)hipHostFree(bh->h_negs);
+  bh->d_pro = nullptr;
+  bh->d_pro_off = nullptr;
+  bh->d_fillcnt = bh->d_negs = nullptr;
+  bh->h_pro = nullptr;
+  bh->h_pro_off = nullptr;
+  bh->h_fillcnt = bh->h_negs = nullptr;
   bh->d_fr = nullptr;
   bh->d_off = nullptr;
   bh->d_work = nullptr;
@@ -813,34 +845,25 @@ static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, Lanes
 // fillcnt[r]), and one wait reads all the counts.  Rounds after the count
 // reached zero change nothing (no hole is left to fill), and a stall is
 // found in the recorded counts, so the outcome is the reference's.
-constexpr int kFillChunk = 8;
-static int fill_loop_n(stsp_ctx** cs, int n, Lanes* L, hipStream_t s) {
+// Hole-filling rounds of every frame until none has holes left
+// (fill_missing.cu:63-115): rounds run in chunks, the hole counts of a chunk
+// are read back once, and a count that stops shrinking is an error as in the
+// reference (fill_missing.cu:97-111).
+static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStream_t s) {
+  const int n = p.nf;
   std::vector<int> prev(n, -1), live(n, 1);
   int chunk = 4;  // most frames finish within a few rounds
-  for (int base = 0; base < 1000; base += chunk, chunk = kFillChunk) {
-    const int rounds = 1000 - base < chunk ? 1000 - base : chunk;
+  for (int base = 0; base < kFillMax; base += chunk, chunk = kFillChunk) {
+    const int rounds = kFillMax - base < chunk ? kFillMax - base : chunk;
+    pro_fill_rounds(p, total, base, rounds, s);
+    HIP_TRY(hipMemcpyAsync(bh->h_fillcnt, p.fillcnt + base * n, sizeof(int) * rounds * n,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     int any = 0;
     for (int i = 0; i < n; i++) {
       if (!live[i]) continue;
-      stsp_ctx* c = cs[i];
-      const hipStream_t ss = lane_of(L, s, i);
-      launch_fill(c->fillcnt, 0, rounds, ss);
       for (int r = 0; r < rounds; r++) {
-        launch_fill_count(c->seg, c->border, c->fillcnt + r, c->W, c->H, ss);
-        launch_fill_phases(c->seg, c->border, c->centers, c->W, c->H, ss);
-      }
-      HIP_TRY(hipMemcpyAsync(c->host + 4, c->fillcnt, sizeof(int) * rounds, hipMemcpyDeviceToHost,
-                             ss));
-      any = 1;
-    }
-    if (!any) return STSP_OK;
-    int rc = lanes_sync(L, s);
-    if (rc) return rc;
-    for (int i = 0; i < n; i++) {
-      if (!live[i]) continue;
-      const int* cnt = cs[i]->host + 4;
-      for (int r = 0; r < rounds; r++) {
-        const int neg = cnt[r];
+        const int neg = bh->h_fillcnt[r * n + i];
         if (neg == 0) {
           live[i] = 0;
           break;
@@ -848,75 +871,92 @@ static int fill_loop_n(stsp_ctx** cs, int n, Lanes* L, hipStream_t s) {
         if (base + r > 0 && neg == prev[i]) return STSP_ERR_FILL;
         prev[i] = neg;
       }
+      any |= live[i];
     }
+    if (!any) return STSP_OK;
   }
   return STSP_OK;
 }
 
 // shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up
-// (prop.cu:677-745) for frames cs[0..n), stage by stage across frames so
-// that the host waits (hole-filling rounds, invalid counts) are shared;
-// frame i runs on lane i.
-static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
+// (prop.cu:677-745) for frames cs[0..n): every stage is one launch over all
+// frames (the ProFrame table), and the host waits only for the hole counts.
+static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* const* imgs,
                            const float* const* flows, const int* const* spix_prev,
-                           stsp_params* prevs, const int* nprev, int* niters_state, Lanes* L,
+                           stsp_params* prevs, const int* nprev, int* niters_state,
                            hipStream_t s) {
-  int rc = lanes_fork(L, s, n);
-  if (rc) return rc;
+  if (n > bh->maxf || n > kProMax) return STSP_ERR_ARG;
+  int* off = bh->h_pro_off;
+  for (int k = 0; k < kProKinds; k++) off[k * (kProMax + 1)] = 0;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
-    const hipStream_t ss = lane_of(L, s, i);
     const int np = nprev[i];
-    rc = ensure_labels(c, 10 * np);  // prop.cu:702
+    int rc = ensure_labels(c, 10 * np);  // prop.cu:702
     if (rc) return rc;
     rc = ensure_prev(c, np);
     if (rc) return rc;
-    launch_flow_pool(flows[i], spix_prev[i], c->npix, c->fsum, c->counts_prev,
-                     prevs[i].mu_shape, c->flow_ds, c->centers, np, ss);
-    launch_shift(spix_prev[i], c->flow_ds, c->counts_prev, c->keys, c->shifted, c->W, c->H, ss);
-    HIP_TRY(hipMemcpyAsync(c->seg, c->shifted, sizeof(int) * c->npix, hipMemcpyDeviceToDevice,
-                           ss));
-    launch_count_labels(c->shifted, c->npix, c->counts_s, np, ss);
+    ProFrame& F = bh->h_pro[i];
+    F.W = c->W;
+    F.H = c->H;
+    F.npix = c->npix;
+    F.np = np;
+    F.nbuf = 10 * np;
+    F.sp_size = c->cfg.sp_size;
+    F.flow = flows[i];
+    F.spix_prev = spix_prev[i];
+    F.mu_shape_prev = prevs[i].mu_shape;
+    F.prior_counts = prevs[i].prior_counts;
+    F.invalid_counts = prevs[i].invalid_counts;
+    memcpy(&F.in, &prevs[i], sizeof(F.in));
+    F.P = c->P;
+    F.fsum = c->fsum;
+    F.counts_prev = c->counts_prev;
+    F.counts_s = c->counts_s;
+    F.counts_f = c->counts_f;
+    F.flow_ds = c->flow_ds;
+    F.centers = c->centers;
+    F.keys = c->keys;
+    F.shifted = c->shifted;
+    F.seg = c->seg;
+    F.parent = c->parent;
+    F.rsize = c->rsize;
+    F.max_spix = c->max_spix;
+    F.border = c->border;
+    F.best = c->best;
+    for (int k = 0; k < kProKinds; k++)
+      off[k * (kProMax + 1) + i + 1] =
+          off[k * (kProMax + 1) + i] + pro_blocks(k, c->W, c->H, np, 10 * np);
   }
-  rc = fill_loop_n(cs, n, L, s);
+  int total[kProKinds];
+  for (int k = 0; k < kProKinds; k++) total[k] = off[k * (kProMax + 1) + n];
+  HIP_TRY(hipMemcpyAsync(bh->d_pro, bh->h_pro, sizeof(ProFrame) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(bh->d_pro_off, off, sizeof(int) * kProKinds * (kProMax + 1),
+                         hipMemcpyHostToDevice, s));
+  Pro p;
+  p.fr = bh->d_pro;
+  p.off = bh->d_pro_off;
+  p.nf = n;
+  p.fillcnt = bh->d_fillcnt;
+  p.negs = bh->d_negs;
+  pro_shift(p, total, s);
+  int rc = fill_loop_pro(bh, p, total, s);
   if (rc) return rc;
+  pro_disconnected(p, total, s);
+  rc = fill_loop_pro(bh, p, total, s);
+  if (rc) return rc;
+  pro_finish(p, total, s);
+  HIP_TRY(hipMemcpyAsync(bh->h_negs, p.negs, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
-    launch_invalidate_disconnected(c->seg, c->parent, c->rsize, c->best, c->W, c->H, nprev[i],
-                                   lane_of(L, s, i));
-  }
-  rc = fill_loop_n(cs, n, L, s);
-  if (rc) return rc;
-  for (int i = 0; i < n; i++) {
-    stsp_ctx* c = cs[i];
-    const hipStream_t ss = lane_of(L, s, i);
-    const int np = nprev[i];
-    launch_count_labels(c->seg, c->npix, c->counts_f, np, ss);
-    launch_prior_update(c->counts_prev, c->counts_s, c->counts_f, prevs[i].prior_counts,
-                        prevs[i].invalid_counts, np, ss);
-    // adaptive iteration count (pyapi.cu:139-149): holes of the shifted map
-    k_set_int<<<1, 1, 0, ss>>>(c->num_neg, 0);
-    launch_fill_count(c->shifted, c->border, c->num_neg, c->W, c->H, ss);
-    HIP_TRY(hipMemcpyAsync(c->host + 3, c->num_neg, sizeof(int), hipMemcpyDeviceToHost, ss));
-  }
-  rc = lanes_sync(L, s);
-  if (rc) return rc;
-  for (int i = 0; i < n; i++) {
-    stsp_ctx* c = cs[i];
-    const hipStream_t ss = lane_of(L, s, i);
     const stsp_config& cfg = c->cfg;
     const int np = nprev[i];
-    const int nbuf = 10 * np;
-    const int ninvalid = c->host[3];
-    float iperc = (float)(ninvalid / (1.0 * c->npix));
+    // adaptive iteration count (pyapi.cu:139-149): holes of the shifted map
+    float iperc = (float)(bh->h_negs[i] / (1.0 * c->npix));
     int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
     if (niters_state) niters_state[i] = adapt;
-    InParams in;
-    memcpy(&in, &prevs[i], sizeof(in));
-    launch_prop_params_init(c->P, in, np, nbuf, cfg.sp_size, c->seg, c->npix, ss);
-    k_set_int<<<1, 1, 0, ss>>>(c->max_spix, np - 1);
     c->img = imgs[i];
-    c->nbuf = nbuf;
+    c->nbuf = 10 * np;
     c->nspix_prev = np;
     c->prop_mode = 1;
     c->split_alpha = cfg.split_alpha;
@@ -926,7 +966,7 @@ static int prop_prologue_n(stsp_ctx** cs, int n, const float* const* imgs,
     c->prev_prior_mu_app = prevs[i].prior_mu_app;
   }
   HIP_TRY(hipGetLastError());
-  return lanes_join(L, s);
+  return STSP_OK;
 }
 
 // compaction (compact_spix.cu:41-122 as called at prop.cu:820-829 and
@@ -1088,8 +1128,8 @@ int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_
   if (!c || !img || !flow || !spix_prev || !prev || !spix_out || !out || !nspix_out ||
       nspix_prev < 1)
     return STSP_ERR_ARG;
-  int rc = prop_prologue_n(&c, 1, &img, &flow, &spix_prev, prev, &nspix_prev, niters_state,
-                           nullptr, s);
+  int rc = prop_prologue_n(&c->self, &c, 1, &img, &flow, &spix_prev, prev, &nspix_prev,
+                           niters_state, s);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
 }
@@ -1186,8 +1226,7 @@ int stsp_prop_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, const float** imgs,
     return STSP_ERR_ARG;
   for (int i = 0; i < n; i++)
     if (nspix_prev[i] < 1) return STSP_ERR_ARG;
-  rc = prop_prologue_n(ctxs, n, imgs, flows, spix_prev, prevs, nspix_prev, niters_state,
-                       &bh->lanes, s);
+  rc = prop_prologue_n(bh, ctxs, n, imgs, flows, spix_prev, prevs, nspix_prev, niters_state, s);
   if (rc) return rc;
   return refine_and_finish(bh, ctxs, n, spix_out, outs, nspix_out, s);
 }

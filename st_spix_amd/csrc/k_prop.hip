@@ -1,34 +1,81 @@
 // Label propagation between frames (reference shift_and_fill.cu, sp_pooling.cu,
-// shift_labels.cu, fill_missing.cu, split_disconnected.cu, relabel.cu,
-// sparams_io.cu).  All collisions and ties are resolved by order-independent
-// integer keys (64-bit atomicMin/atomicMax), so the result does not depend on
-// thread scheduling.
+// shift_labels.cu, fill_missing.cu, split_disconnected.cu, sparams_io.cu).
+// All collisions and ties are resolved by order-independent integer keys
+// (64-bit atomicMin/atomicMax), so the result does not depend on thread
+// scheduling.
+//
+// Every kernel covers all frames of a batch: block b belongs to the frame f
+// with off[kind][f] <= b < off[kind][f + 1], so a stage is one launch however
+// many frames there are, and a block never straddles two frames.
 #include "kernels.h"
 
 namespace stsp {
 
-// ---------------------------------------------------------------------------
-// per-label mean flow (run_sp_downsample, sp_pooling.cu:27-71) and shifted
-// centres (shift_and_fill.cu:144-150); rule R1 (2^-24 fixed point).
-// ---------------------------------------------------------------------------
-__global__ void k_flow_clear(int64_t* fsum, int* counts, int n) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  fsum[2 * k] = fsum[2 * k + 1] = 0;
-  counts[k] = 0;
+__device__ __forceinline__ int pro_frame(const Pro& p, int kind, int b) {
+  const int* off = p.off + kind * (kProMax + 1);
+  int lo = 0, hi = p.nf - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ int pro_local(const Pro& p, int kind, int f, int b) {
+  return b - p.off[kind * (kProMax + 1) + f];
 }
 
-__global__ void k_flow_accum(const float* __restrict__ flow, const int* __restrict__ spix,
-                             int npix, int64_t* fsum, int* counts, int n) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+// per-pixel kernels: f = frame, t = pixel index (may be >= npix)
+#define PRO_PIX                                              \
+  const int f = pro_frame(p, kProPix, blockIdx.x);           \
+  const ProFrame& F = p.fr[f];                               \
+  const int t = pro_local(p, kProPix, f, blockIdx.x) * kBlock + threadIdx.x;
+// per-label kernels over the previous labels: k < np
+#define PRO_LAB                                              \
+  const int f = pro_frame(p, kProLab, blockIdx.x);           \
+  const ProFrame& F = p.fr[f];                               \
+  const int k = pro_local(p, kProLab, f, blockIdx.x) * kBlock + threadIdx.x;
+
+int pro_blocks(int kind, int W, int H, int np, int nbuf) {
+  switch (kind) {
+    case kProPix: return div_up((long long)W * H, kBlock);
+    case kProLab: return div_up(np, kBlock);
+    case kProBuf: return div_up(nbuf, kBlock);
+    case kProTile: return div_up(W, 32) * div_up(H, 32);
+    default: return div_up((long long)((W + 1) / 2) * ((H + 1) / 2), kBlock);  // kProHalf
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-label mean flow (run_sp_downsample, sp_pooling.cu:27-71) and shifted
+// centres (shift_and_fill.cu:144-150); rule R1 (2^-24 fixed point).  The
+// clear also zeroes the label counts, the best-region keys of the
+// connectivity pass and the hole count, and sets max_spix (prop.cu:736).
+// ---------------------------------------------------------------------------
+__global__ void kp_clear(Pro p) {
+  PRO_LAB
+  if (k == 0) {
+    p.negs[f] = 0;
+    *F.max_spix = F.np - 1;
+  }
+  if (k >= F.np) return;
+  F.fsum[2 * k] = F.fsum[2 * k + 1] = 0;
+  F.counts_prev[k] = 0;
+  F.counts_s[k] = 0;
+  F.counts_f[k] = 0;
+  F.best[k] = 0ull;
+}
+
+__global__ void kp_flow_accum(Pro p) {
+  PRO_PIX
   int L = -1;
   int64_t u = 0, v = 0;
-  if (t < npix) {
-    L = spix[t];
-    if (L >= n) L = -1;
+  if (t < F.npix) {
+    L = F.spix_prev[t];
+    if (L >= F.np) L = -1;
     if (L >= 0) {
-      u = __double2ll_rn((double)flow[2 * t] * kFxFlow);
-      v = __double2ll_rn((double)flow[2 * t + 1] * kFxFlow);
+      u = __double2ll_rn((double)F.flow[2 * t] * kFxFlow);
+      v = __double2ll_rn((double)F.flow[2 * t + 1] * kFxFlow);
     }
   }
   int run_end;
@@ -36,97 +83,132 @@ __global__ void k_flow_accum(const float* __restrict__ flow, const int* __restri
   u = run_sum64(u, run_end);
   v = run_sum64(v, run_end);
   if (head && L >= 0) {
-    atomic_add_i64(fsum + 2 * L, u);
-    atomic_add_i64(fsum + 2 * L + 1, v);
-    atomicAdd(counts + L, run_end - (int)(threadIdx.x & 63));
+    atomic_add_i64(F.fsum + 2 * L, u);
+    atomic_add_i64(F.fsum + 2 * L + 1, v);
+    atomicAdd(F.counts_prev + L, run_end - (int)(threadIdx.x & 63));
   }
 }
 
-__global__ void k_flow_norm(const int64_t* fsum, const int* counts, const double* mu_shape,
-                            float* flow_ds, double* centers, int n) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+__global__ void kp_flow_norm(Pro p) {
+  PRO_LAB
+  if (k >= F.np) return;
   for (int c = 0; c < 2; c++) {
-    double sum = (double)fsum[2 * k + c] * (1.0 / kFxFlow);
-    float v = counts[k] > 0 ? (float)(sum / (1. * counts[k])) : 0.0f;
-    flow_ds[2 * k + c] = v;
-    centers[2 * k + c] = mu_shape[2 * k + c] + (double)v;
+    double sum = (double)F.fsum[2 * k + c] * (1.0 / kFxFlow);
+    float v = F.counts_prev[k] > 0 ? (float)(sum / (1. * F.counts_prev[k])) : 0.0f;
+    F.flow_ds[2 * k + c] = v;
+    F.centers[2 * k + c] = F.mu_shape_prev[2 * k + c] + (double)v;
   }
-}
-
-void launch_flow_pool(const float* flow, const int* spix, int npix, int64_t* fsum, int* counts,
-                      const double* mu_shape, float* flow_ds, double* centers, int n,
-                      hipStream_t s) {
-  k_flow_clear<<<div_up(n, kBlock), kBlock, 0, s>>>(fsum, counts, n);
-  k_flow_accum<<<div_up(npix, kBlock), kBlock, 0, s>>>(flow, spix, npix, fsum, counts, n);
-  k_flow_norm<<<div_up(n, kBlock), kBlock, 0, s>>>(fsum, counts, mu_shape, flow_ds, centers, n);
 }
 
 // ---------------------------------------------------------------------------
 // label shift with collision resolution (shift_labels.cu:32-164), rule R5:
 // the destination keeps the lexicographic minimum of (source size, label).
 // ---------------------------------------------------------------------------
-__global__ void k_u64_fill(unsigned long long* p, unsigned long long v, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+__global__ void kp_keys_fill(Pro p) {
+  PRO_PIX
+  if (t < F.npix) F.keys[t] = ~0ull;
 }
 
-__global__ void k_shift(const int* __restrict__ spix, const float* __restrict__ flow_ds,
-                        const int* __restrict__ sizes, unsigned long long* keys, int W, int H) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= W * H) return;
-  int L = spix[t];
-  if (L < 0) return;
+__global__ void kp_shift(Pro p) {
+  PRO_PIX
+  if (t >= F.npix) return;
+  const int W = F.W, H = F.H;
+  int L = F.spix_prev[t];
+  if (L < 0 || L >= F.np) return;
   int h = t / W, w = t % W;
-  int fw = (int)roundf(flow_ds[2 * L]);
-  int fh = (int)roundf(flow_ds[2 * L + 1]);
+  int fw = (int)roundf(F.flow_ds[2 * L]);
+  int fh = (int)roundf(F.flow_ds[2 * L + 1]);
   int wd = w + fw, hd = h + fh;
   if (wd < 0 || wd >= W || hd < 0 || hd >= H) return;
-  unsigned long long key = ((unsigned long long)(unsigned)sizes[L] << 32) | (unsigned)L;
-  atomicMin(keys + hd * W + wd, key);
+  unsigned long long key = ((unsigned long long)(unsigned)F.counts_prev[L] << 32) | (unsigned)L;
+  atomicMin(F.keys + hd * W + wd, key);
 }
 
-__global__ void k_shift_decode(const unsigned long long* keys, int* shifted, int npix) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  unsigned long long k = keys[t];
-  shifted[t] = (k == ~0ull) ? -1 : (int)(unsigned)(k & 0xffffffffull);
+// decodes the shifted map into both the kept copy and the working labels
+__global__ void kp_shift_decode(Pro p) {
+  PRO_PIX
+  if (t >= F.npix) return;
+  unsigned long long k = F.keys[t];
+  const int v = (k == ~0ull) ? -1 : (int)(unsigned)(k & 0xffffffffull);
+  F.shifted[t] = v;
+  F.seg[t] = v;
 }
 
-void launch_shift(const int* spix, const float* flow_ds, const int* sizes,
-                  unsigned long long* keys, int* shifted, int W, int H, hipStream_t s) {
-  int npix = W * H;
-  k_u64_fill<<<div_up(npix, kBlock), kBlock, 0, s>>>(keys, ~0ull, npix);
-  k_shift<<<div_up(npix, kBlock), kBlock, 0, s>>>(spix, flow_ds, sizes, keys, W, H);
-  k_shift_decode<<<div_up(npix, kBlock), kBlock, 0, s>>>(keys, shifted, npix);
+// per-label pixel counts (shift_and_fill.cu:223-322): of the shifted map
+// (which == 0) or of the filled labels (which == 1)
+__global__ void kp_count_labels(Pro p, int which) {
+  PRO_PIX
+  const int* seg = which == 0 ? F.shifted : F.seg;
+  int* counts = which == 0 ? F.counts_s : F.counts_f;
+  int k = -1;
+  if (t < F.npix) {
+    k = seg[t];
+    if (k >= F.np) k = -1;
+  }
+  int run_end;
+  bool head = wave_run(k, run_end);
+  if (head && k >= 0) atomicAdd(counts + k, run_end - (int)(threadIdx.x & 63));
+}
+
+// holes of the shifted map (the adaptive iteration count, pyapi.cu:139-149)
+__global__ void kp_count_holes(Pro p) {
+  PRO_PIX
+  const bool neg = t < F.npix && F.shifted[t] < 0;
+  const unsigned long long b = __ballot(neg);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(p.negs + f, __popcll(b));
+}
+
+void pro_shift(const Pro& p, const int* total, hipStream_t s) {
+  kp_clear<<<total[kProLab], kBlock, 0, s>>>(p);
+  kp_flow_accum<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_flow_norm<<<total[kProLab], kBlock, 0, s>>>(p);
+  kp_keys_fill<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_shift<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_shift_decode<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_count_labels<<<total[kProPix], kBlock, 0, s>>>(p, 0);
+  kp_count_holes<<<total[kProPix], kBlock, 0, s>>>(p);
 }
 
 // ---------------------------------------------------------------------------
-// hole filling (fill_missing.cu:35-130, 288-441, 617-647)
+// hole filling (fill_missing.cu:35-130, 288-441, 617-647).  Round r counts
+// the holes and marks the border (a valid 4-neighbour) at its start, then
+// fills border holes in four checkerboard phases.  A frame whose count was
+// zero in the round before is finished: its blocks return at once.
 // ---------------------------------------------------------------------------
-__global__ void k_fill_border(const int* __restrict__ seg, uint8_t* border, int* num_neg, int W,
-                              int H) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void kp_fill_zero(Pro p, int base, int rounds) {
+  const int n = rounds * p.nf;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nf + i] = 0;
+}
+
+__global__ void kp_fill_border(Pro p, int r) {
+  PRO_PIX
+  if (r > 0 && p.fillcnt[(r - 1) * p.nf + f] == 0) return;  // block-uniform
   bool neg = false;
-  if (idx < W * H) {
-    int x = idx % W, y = idx / W;
-    int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
-    int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
-    border[idx] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
-    neg = seg[idx] < 0;
+  if (t < F.npix) {
+    const int W = F.W, H = F.H;
+    const int* seg = F.seg;
+    int x = t % W, y = t / W;
+    int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
+    int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
+    F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
+    neg = seg[t] < 0;
   }
   unsigned long long b = __ballot(neg);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(num_neg, __popcll(b));
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(p.fillcnt + r * p.nf + f, __popcll(b));
 }
 
-__global__ void k_fill_phase(int* seg, const uint8_t* __restrict__ border,
-                             const double* __restrict__ centers, int W, int H, int xm, int ym) {
+__global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {
+  const int f = pro_frame(p, kProHalf, blockIdx.x);
+  if (p.fillcnt[r * p.nf + f] == 0) return;
+  const ProFrame& F = p.fr[f];
+  const int W = F.W, H = F.H;
   int hw = (W - xm + 1) / 2, hh = (H - ym + 1) / 2;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = pro_local(p, kProHalf, f, blockIdx.x) * kBlock + threadIdx.x;
   if (i >= hw * hh) return;
+  int* seg = F.seg;
   int x = 2 * (i % hw) + xm, y = 2 * (i / hw) + ym;
   int idx = y * W + x;
-  if (!border[idx] || seg[idx] != -1) return;
+  if (!F.border[idx] || seg[idx] != -1) return;
   int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
   int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
   const int cand[4] = {N, S, Wl, E};
@@ -139,8 +221,9 @@ __global__ void k_fill_phase(int* seg, const uint8_t* __restrict__ border,
     for (int q = 0; q < j; q++) dup |= cand[q] == L;
     if (dup) continue;
     // isotropic_space, fill_missing.cu:140-153
-    float dx = (float)(((double)(1.0f * (float)x) - centers[2 * L]) / (double)(1.0f * (float)W));
-    float dy = (float)(((double)(1.0f * (float)y) - centers[2 * L + 1]) / (double)(1.0f * (float)H));
+    float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
+    float dy =
+        (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
     float sim = -dx * dx - dy * dy;
     if (sim > best) {
       best = sim;
@@ -150,19 +233,14 @@ __global__ void k_fill_phase(int* seg, const uint8_t* __restrict__ border,
   seg[idx] = lab;
 }
 
-void launch_fill_count(const int* seg, uint8_t* border, int* num_neg, int W, int H,
-                       hipStream_t s) {
-  k_fill_border<<<div_up((long long)W * H, kBlock), kBlock, 0, s>>>(seg, border, num_neg, W, H);
-}
-
-void launch_fill_phases(int* seg, const uint8_t* border, const double* centers, int W, int H,
-                        hipStream_t s) {
-  for (int xm = 0; xm < 2; xm++)
-    for (int ym = 0; ym < 2; ym++) {
-      int hw = (W - xm + 1) / 2, hh = (H - ym + 1) / 2;
-      k_fill_phase<<<div_up((long long)hw * hh, kBlock), kBlock, 0, s>>>(seg, border, centers, W,
-                                                                         H, xm, ym);
-    }
+void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s) {
+  kp_fill_zero<<<1, 256, 0, s>>>(p, base, rounds);
+  for (int r = base; r < base + rounds; r++) {
+    kp_fill_border<<<total[kProPix], kBlock, 0, s>>>(p, r);
+    for (int xm = 0; xm < 2; xm++)
+      for (int ym = 0; ym < 2; ym++)
+        kp_fill_phase<<<total[kProHalf], kBlock, 0, s>>>(p, r, xm, ym);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -230,13 +308,17 @@ __device__ __forceinline__ void lds_union(int* P, int a, int b) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_cc_tile(const int* __restrict__ seg, int* parent,
-                                                 int* rsize, int W, int H) {
+__global__ void __launch_bounds__(256) kp_cc_tile(Pro p) {
   constexpr int T = kCcTile, N = T * T;
   __shared__ int lab[N];
   __shared__ int P[N];
+  const int f = pro_frame(p, kProTile, blockIdx.x);
+  const ProFrame& F = p.fr[f];
+  const int W = F.W, H = F.H;
+  const int* __restrict__ seg = F.seg;
+  const int tile = pro_local(p, kProTile, f, blockIdx.x);
   const int tiles_x = (W + T - 1) / T;
-  const int tx0 = (blockIdx.x % tiles_x) * T, ty0 = (blockIdx.x / tiles_x) * T;
+  const int tx0 = (tile % tiles_x) * T, ty0 = (tile / tiles_x) * T;
   const int tid = threadIdx.x;
   int v[N / 256];
 #pragma unroll
@@ -271,19 +353,23 @@ __global__ void __launch_bounds__(256) k_cc_tile(const int* __restrict__ seg, in
     const int gx = tx0 + i % T, gy = ty0 + i / T;
     if (gx >= W || gy >= H) continue;
     const int r = lds_find(P, i);
-    parent[gy * W + gx] = (ty0 + r / T) * W + tx0 + r % T;
-    rsize[gy * W + gx] = 0;
+    F.parent[gy * W + gx] = (ty0 + r / T) * W + tx0 + r % T;
+    F.rsize[gy * W + gx] = 0;
   }
 }
 
 // Stage 2: the neighbour pairs that cross a tile edge (each pixel's W, N, NW
 // and NE neighbours; a pair is owned by its later pixel) are linked in the
 // global forest, whose chains now run only between tile roots.
-__global__ void __launch_bounds__(128) k_cc_seams(const int* __restrict__ seg, int* parent, int W,
-                                                  int H) {
+__global__ void __launch_bounds__(128) kp_cc_seams(Pro p) {
   constexpr int T = kCcTile;
+  const int f = pro_frame(p, kProTile, blockIdx.x);
+  const ProFrame& F = p.fr[f];
+  const int W = F.W, H = F.H;
+  const int* __restrict__ seg = F.seg;
+  const int tile = pro_local(p, kProTile, f, blockIdx.x);
   const int tiles_x = (W + T - 1) / T;
-  const int tx0 = (blockIdx.x % tiles_x) * T, ty0 = (blockIdx.x / tiles_x) * T;
+  const int tx0 = (tile % tiles_x) * T, ty0 = (tile / tiles_x) * T;
   const int t = threadIdx.x;
   if (t >= 3 * T) return;
   // t < T: top row; T <= t < 2T: left column; 2T <= t: right column
@@ -295,6 +381,7 @@ __global__ void __launch_bounds__(128) k_cc_seams(const int* __restrict__ seg, i
   if (L < 0) return;
   const bool top = ly == 0, left = lx == 0, right = lx == T - 1;
   if (t >= T && top) return;  // the top row owns its corners
+  int* parent = F.parent;
   if (left && x > 0 && seg[i - 1] == L) uf_union(parent, i, i - 1);
   if (y > 0) {
     if (top && seg[i - W] == L) uf_union(parent, i, i - W);
@@ -306,97 +393,67 @@ __global__ void __launch_bounds__(128) k_cc_seams(const int* __restrict__ seg, i
 // Points every pixel at its root and counts region sizes; neighbouring
 // pixels of a wave mostly share a root, so the counts are added once per run
 // of equal roots.
-__global__ void k_cc_compress(int* parent, int* rsize, int npix) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void kp_cc_compress(Pro p) {
+  PRO_PIX
   int r = -1;
-  if (i < npix) {
-    r = uf_find(parent, i);
-    parent[i] = r;
+  if (t < F.npix) {
+    r = uf_find(F.parent, t);
+    F.parent[t] = r;
   }
   int run_end;
   const bool head = wave_run(r, run_end);
-  if (head && r >= 0) atomicAdd(rsize + r, run_end - (int)(threadIdx.x & 63));
+  if (head && r >= 0) atomicAdd(F.rsize + r, run_end - (int)(threadIdx.x & 63));
 }
 
-__global__ void k_cc_best(const int* __restrict__ seg, const int* __restrict__ root,
-                          const int* __restrict__ rsize, unsigned long long* best, int npix,
-                          int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix || root[i] != i) return;
-  int L = seg[i];
-  if (L < 0 || L >= n) return;
-  unsigned long long key = ((unsigned long long)(unsigned)rsize[i] << 32) |
-                           (unsigned long long)(0xffffffffu - (unsigned)i);
-  atomicMax(best + L, key);
+__global__ void kp_cc_best(Pro p) {
+  PRO_PIX
+  if (t >= F.npix || F.parent[t] != t) return;
+  int L = F.seg[t];
+  if (L < 0 || L >= F.np) return;
+  unsigned long long key = ((unsigned long long)(unsigned)F.rsize[t] << 32) |
+                           (unsigned long long)(0xffffffffu - (unsigned)t);
+  atomicMax(F.best + L, key);
 }
 
-__global__ void k_cc_apply(int* seg, const int* __restrict__ root,
-                           const unsigned long long* __restrict__ best, int npix, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  int L = seg[i];
-  if (L < 0 || L >= n) return;
-  unsigned keep = 0xffffffffu - (unsigned)(best[L] & 0xffffffffull);
-  if ((unsigned)root[i] != keep) seg[i] = -1;
+__global__ void kp_cc_apply(Pro p) {
+  PRO_PIX
+  if (t >= F.npix) return;
+  int L = F.seg[t];
+  if (L < 0 || L >= F.np) return;
+  unsigned keep = 0xffffffffu - (unsigned)(F.best[L] & 0xffffffffull);
+  if ((unsigned)F.parent[t] != keep) F.seg[t] = -1;
 }
 
-void launch_invalidate_disconnected(int* seg, int* parent, int* rsize, unsigned long long* best,
-                                    int W, int H, int n, hipStream_t s) {
-  int npix = W * H;
-  const int tiles = div_up(W, kCcTile) * div_up(H, kCcTile);
-  k_u64_fill<<<div_up(n, kBlock), kBlock, 0, s>>>(best, 0ull, n);
-  k_cc_tile<<<tiles, 256, 0, s>>>(seg, parent, rsize, W, H);
-  k_cc_seams<<<tiles, 128, 0, s>>>(seg, parent, W, H);
-  k_cc_compress<<<div_up(npix, kBlock), kBlock, 0, s>>>(parent, rsize, npix);
-  k_cc_best<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, parent, rsize, best, npix, n);
-  k_cc_apply<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, parent, best, npix, n);
+void pro_disconnected(const Pro& p, const int* total, hipStream_t s) {
+  kp_cc_tile<<<total[kProTile], 256, 0, s>>>(p);
+  kp_cc_seams<<<total[kProTile], 128, 0, s>>>(p);
+  kp_cc_compress<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_cc_best<<<total[kProPix], kBlock, 0, s>>>(p);
+  kp_cc_apply<<<total[kProPix], kBlock, 0, s>>>(p);
 }
 
 // ---------------------------------------------------------------------------
-// per-label pixel counts, invalid fractions and prior counts
-// (shift_and_fill.cu:223-322)
-// ---------------------------------------------------------------------------
-__global__ void k_count_labels(const int* __restrict__ seg, int npix, int* counts, int n) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int k = -1;
-  if (t < npix) {
-    k = seg[t];
-    if (k >= n) k = -1;
-  }
-  int run_end;
-  bool head = wave_run(k, run_end);
-  if (head && k >= 0) atomicAdd(counts + k, run_end - (int)(threadIdx.x & 63));
-}
-
-void launch_count_labels(const int* seg, int npix, int* counts, int n, hipStream_t s) {
-  launch_fill(counts, 0, n, s);
-  k_count_labels<<<div_up(npix, kBlock), kBlock, 0, s>>>(seg, npix, counts, n);
-}
-
-__global__ void k_prior_update(const int* counts_prev, const int* counts_s, const int* counts_f,
-                               float* prior_counts, float* invalid_counts, int n) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int f = counts_f[k], sh = counts_s[k];
-  invalid_counts[k] = (float)((double)(f - sh) / ((double)(float)f + 1e-8));
-  int c = counts_prev[k];
-  if (c > 0) prior_counts[k] = (float)fmin(fmax(1.0 * c, 100.0), 10000.0);
-}
-
-void launch_prior_update(const int* counts_prev, const int* counts_s, const int* counts_f,
-                         float* prior_counts, float* invalid_counts, int n, hipStream_t s) {
-  k_prior_update<<<div_up(n, kBlock), kBlock, 0, s>>>(counts_prev, counts_s, counts_f,
-                                                      prior_counts, invalid_counts, n);
-}
-
-// ---------------------------------------------------------------------------
+// invalid fractions and prior counts (shift_and_fill.cu:223-322), then
 // get_vectors_as_params + mark_active (sparams_io.cu:123-140, 352-501;
 // init_sparams.cu:195-286)
 // ---------------------------------------------------------------------------
-__global__ void k_prop_params_init(Params P, InParams in, int nspix, int nbuf, int sp_size) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nbuf) return;
-  if (k < nspix) {
+__global__ void kp_prior_update(Pro p) {
+  PRO_LAB
+  if (k >= F.np) return;
+  int fc = F.counts_f[k], sh = F.counts_s[k];
+  F.invalid_counts[k] = (float)((double)(fc - sh) / ((double)(float)fc + 1e-8));
+  int c = F.counts_prev[k];
+  if (c > 0) F.prior_counts[k] = (float)fmin(fmax(1.0 * c, 100.0), 10000.0);
+}
+
+__global__ void kp_params_init(Pro p) {
+  const int f = pro_frame(p, kProBuf, blockIdx.x);
+  const ProFrame& F = p.fr[f];
+  const int k = pro_local(p, kProBuf, f, blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= F.nbuf) return;
+  const Params& P = F.P;
+  const InParams& in = F.in;
+  if (k < F.np) {
     for (int c = 0; c < 3; c++) {
       P.mu_app[3 * k + c] = in.mu_app[3 * k + c];
       P.prior_mu_app[3 * k + c] = in.prior_mu_app[3 * k + c];
@@ -426,9 +483,9 @@ __global__ void k_prop_params_init(Params P, InParams in, int nspix, int nbuf, i
       P.sample_sigma_shape[3 * k + c] = 0.0;
       P.prior_icov[3 * k + c] = 0.0;
     }
-    P.prior_sigma_shape[3 * k] = 1.0 / sp_size;
+    P.prior_sigma_shape[3 * k] = 1.0 / F.sp_size;
     P.prior_sigma_shape[3 * k + 1] = 0;
-    P.prior_sigma_shape[3 * k + 2] = 1.0 / sp_size;
+    P.prior_sigma_shape[3 * k + 2] = 1.0 / F.sp_size;
     for (int c = 0; c < 2; c++) {
       P.mu_shape[2 * k + c] = 0.0;
       P.prior_mu_shape[2 * k + c] = 0.0;
@@ -443,20 +500,21 @@ __global__ void k_prop_params_init(Params P, InParams in, int nspix, int nbuf, i
   P.prop[k] = 0;
 }
 
-__global__ void k_mark_active_px(const int* __restrict__ seg, int npix, Params P, int nbuf) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npix) return;
-  int L = seg[t];
-  if (L >= 0 && L < nbuf) {
-    P.valid[L] = 1;
-    P.prop[L] = 1;
+__global__ void kp_mark_active(Pro p) {
+  PRO_PIX
+  if (t >= F.npix) return;
+  int L = F.seg[t];
+  if (L >= 0 && L < F.nbuf) {
+    F.P.valid[L] = 1;
+    F.P.prop[L] = 1;
   }
 }
 
-void launch_prop_params_init(Params P, InParams in, int nspix, int nbuf, int sp_size,
-                             const int* filled, int npix, hipStream_t s) {
-  k_prop_params_init<<<div_up(nbuf, kBlock), kBlock, 0, s>>>(P, in, nspix, nbuf, sp_size);
-  k_mark_active_px<<<div_up(npix, kBlock), kBlock, 0, s>>>(filled, npix, P, nbuf);
+void pro_finish(const Pro& p, const int* total, hipStream_t s) {
+  kp_count_labels<<<total[kProPix], kBlock, 0, s>>>(p, 1);
+  kp_prior_update<<<total[kProLab], kBlock, 0, s>>>(p);
+  kp_params_init<<<total[kProBuf], kBlock, 0, s>>>(p);
+  kp_mark_active<<<total[kProPix], kBlock, 0, s>>>(p);
 }
 
 }  // namespace stsp

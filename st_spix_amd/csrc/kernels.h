@@ -48,23 +48,45 @@ size_t compact_tmp_bytes(int n);
 void launch_export(Params P, OutParams o, const int* nspix, const int* alive, int nmax,
                    hipStream_t s);
 
-// k_prop.hip
-void launch_flow_pool(const float* flow, const int* spix, int npix, int64_t* fsum, int* counts,
-                      const double* mu_shape, float* flow_ds, double* centers, int n,
-                      hipStream_t s);
-void launch_shift(const int* spix, const float* flow_ds, const int* sizes,
-                  unsigned long long* keys, int* shifted, int W, int H, hipStream_t s);
-void launch_fill_count(const int* seg, uint8_t* border, int* num_neg, int W, int H,
-                       hipStream_t s);
-void launch_fill_phases(int* seg, const uint8_t* border, const double* centers, int W, int H,
-                        hipStream_t s);
-void launch_invalidate_disconnected(int* seg, int* parent, int* rsize, unsigned long long* best,
-                                    int W, int H, int n, hipStream_t s);
-void launch_count_labels(const int* seg, int npix, int* counts, int n, hipStream_t s);
-void launch_prior_update(const int* counts_prev, const int* counts_s, const int* counts_f,
-                         float* prior_counts, float* invalid_counts, int n, hipStream_t s);
-void launch_prop_params_init(Params P, InParams in, int nspix, int nbuf, int sp_size,
-                             const int* filled, int npix, hipStream_t s);
+// k_prop.hip: the propagation prologue of every frame of a batch, one
+// launch per stage.  Each frame's pointers and sizes are one ProFrame entry
+// of a device table; ProOff holds, per block kind, the first block of each
+// frame (prefix sums, kProMax + 1 entries each).
+constexpr int kProMax = 64;     // frames per batch
+constexpr int kFillMax = 1000;  // hole-filling rounds (fill_missing.cu break_iter)
+struct ProFrame {
+  int W, H, npix, np, nbuf, sp_size;
+  const float* flow;       // [npix, 2]
+  const int* spix_prev;    // [npix]
+  const double* mu_shape_prev;
+  float* prior_counts;     // previous parameters, updated in place
+  float* invalid_counts;
+  InParams in;             // previous parameters
+  Params P;                // working parameters (nbuf slots)
+  int64_t* fsum;
+  int *counts_prev, *counts_s, *counts_f;
+  float* flow_ds;
+  double* centers;
+  unsigned long long* keys;
+  int *shifted, *seg, *parent, *rsize, *max_spix;
+  uint8_t* border;
+  unsigned long long* best;
+};
+enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProKinds };
+struct Pro {
+  const ProFrame* fr;  // device table
+  const int* off;      // [kProKinds][kProMax + 1]
+  int nf;
+  int* fillcnt;        // [kFillMax][nf]: holes at the start of each round
+  int* negs;           // [nf]: holes of the shifted map
+};
+// host side: block counts per kind for a frame
+int pro_blocks(int kind, int W, int H, int np, int nbuf);
+// the whole prologue except the hole-filling rounds
+void pro_shift(const Pro& p, const int* total, hipStream_t s);
+void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s);
+void pro_disconnected(const Pro& p, const int* total, hipStream_t s);
+void pro_finish(const Pro& p, const int* total, hipStream_t s);
 // colour preprocessing (k_color.hip)
 void launch_rescale_u8(const uint8_t* in, float* out, long long n, float scale, hipStream_t s);
 void launch_rgb_to_lab(const float* bgr, float* lab, long long npix, hipStream_t s);
