@@ -865,42 +865,48 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
   }
 }
 
-// Boxes beyond the wave path: one workgroup per label, thread t holding rows
-// t + 256c (c < RC) of the box, WD words each, in registers.  Rows cross
-// waves (and chunks) through LDS: each wave publishes the first and last row
-// of every chunk per level, and which of its units (chunk, word, seed) grew,
-// all double-buffered by level parity, so one barrier ends a level.  A unit
-// is expanded only if it or an adjacent unit grew in the level before:
-// otherwise nothing it reads has changed and its new front is empty.  Up to
-// 256 * kRowsChunks rows and 64 * kRowsWords columns; larger boxes use LDS
-// bitsets (up to kBlkWords words) or per-pixel distances in global memory.
-constexpr int kRowsChunks = 4, kRowsWords = 6, kBlkWords = 2048;
+// Boxes beyond the wave path: one workgroup per label, in bands of rows.
+// Band b = 4c + wave holds rows 56b - 4 .. 56b + 59 of the box in the 64
+// lanes of wave `wave` (chunk c), WD words per row in registers: lanes 4..59
+// are its own rows, lanes 0..3 and 60..63 copies of the neighbouring bands'
+// rows.  Four levels run on registers only (vertical neighbours by DPP wave
+// shifts); the copies go stale from the outside in by one row per level, so
+// the own rows stay exact.  Then the bands trade their outer own rows through
+// LDS (double-buffered by round parity) and one barrier ends the round.  The
+// search ends after a round in which no own row grew: the level before that
+// round was already a fixed point.  Up to 4 * kBandChunks bands of 56 rows and
+// 64 * kBandWords columns; larger boxes use LDS bitsets (up to kBlkWords
+// words) or per-pixel distances in global memory.
+constexpr int kHalo = 4, kOwn = 64 - 2 * kHalo, kBandChunks = 2, kBandWords = 6;
+constexpr int kBlkWords = 2048;
 
-struct RowsEdge {
-  uint64_t v[2][kRowsChunks][4][2][2][kRowsWords];  // [buf][chunk][wave][first|last][seed][word]
-  uint32_t g[2][4];  // [buf][wave]: bit 2 (c WD + w) + seed set if that unit grew
+struct BandEdge {
+  uint64_t v[2][4 * kBandChunks][2][kHalo][2][kBandWords];  // [buf][band][top|bottom][row][seed][word]
+  uint32_t g[2][4];                                         // [buf][wave]: own rows grew
 };
 
-template <int RC, int WD>
-__device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int k, int bx0,
-                                         int by0, int bw, int bh, int s1, int s2, int k2,
-                                         int* __restrict__ halves, RowsEdge& E) {
+template <int RB, int WD>
+__device__ __forceinline__ void bfs_bands(const int* __restrict__ seg, int W, int k, int bx0,
+                                          int by0, int bw, int bh, int s1, int s2, int k2,
+                                          int* __restrict__ halves, BandEdge& E) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  uint64_t M[RC][WD], V1[RC][WD], V2[RC][WD], NK[RC][WD];
+  uint64_t M[RB][WD], V1[RB][WD], V2[RB][WD], NK[RB][WD];
+  const bool own = lane >= kHalo && lane < 64 - kHalo;
 #pragma unroll
-  for (int c = 0; c < RC; c++) {
-    const int nrows = min(64, bh - 256 * c - 64 * wv);  // wave-uniform, may be <= 0
+  for (int c = 0; c < RB; c++) {
+    const int row0 = kOwn * (4 * c + wv) - kHalo;  // row of lane 0 (wave-uniform)
 #pragma unroll
     for (int w = 0; w < WD; w++) {
       M[c][w] = V1[c][w] = V2[c][w] = NK[c][w] = 0;
       const int col = 64 * w + lane;
-      for (int r0 = 0; r0 < nrows; r0 += 16) {
-        int v[16];
+      if (row0 >= bh) continue;
+      for (int r0 = 0; r0 < 64; r0 += 16) {
+        int v[16];  // 16 row loads in flight, then 16 ballots
 #pragma unroll
-        for (int j = 0; j < 16; j++)
-          v[j] = (r0 + j < nrows && col < bw)
-                     ? seg[(by0 + 256 * c + 64 * wv + r0 + j) * W + bx0 + col]
-                     : -1;
+        for (int j = 0; j < 16; j++) {
+          const int y = row0 + r0 + j;
+          v[j] = (y >= 0 && y < bh && col < bw) ? seg[(by0 + y) * W + bx0 + col] : -1;
+        }
 #pragma unroll
         for (int j = 0; j < 16; j++) {
           const uint64_t word = __ballot(v[j] == k);
@@ -911,123 +917,94 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
   }
   {
     const int y1 = s1 / W - by0, x1 = s1 % W - bx0;
-    const int y2 = s2 >= 0 ? s2 / W - by0 : -1, x2 = s2 >= 0 ? s2 % W - bx0 : 0;
+    const int y2 = s2 >= 0 ? s2 / W - by0 : -1000, x2 = s2 >= 0 ? s2 % W - bx0 : 0;
 #pragma unroll
-    for (int c = 0; c < RC; c++) {
+    for (int c = 0; c < RB; c++) {
+      const int y = kOwn * (4 * c + wv) - kHalo + lane;
 #pragma unroll
       for (int w = 0; w < WD; w++) {
-        if (y1 == 256 * c + tid && (x1 >> 6) == w) V1[c][w] |= 1ull << (x1 & 63);
-        if (y2 == 256 * c + tid && (x2 >> 6) == w) V2[c][w] |= 1ull << (x2 & 63);
+        if (y1 == y && (x1 >> 6) == w) V1[c][w] |= 1ull << (x1 & 63);
+        if (y2 == y && (x2 >> 6) == w) V2[c][w] |= 1ull << (x2 & 63);
       }
     }
   }
-#pragma unroll
-  for (int c = 0; c < RC; c++) {
-#pragma unroll
-    for (int w = 0; w < WD; w++) {
-      if (lane == 0) {
-        E.v[0][c][wv][0][0][w] = V1[c][w];
-        E.v[0][c][wv][0][1][w] = V2[c][w];
-      }
-      if (lane == 63) {
-        E.v[0][c][wv][1][0][w] = V1[c][w];
-        E.v[0][c][wv][1][1][w] = V2[c][w];
-      }
-    }
-  }
-  if (tid < 4) E.g[0][tid] = 0xffffffffu;  // level 0 expands every unit
-  __syncthreads();
-  // unit bits: (c WD + w) 2 + seed; a unit's horizontal neighbours are the
-  // same chunk's words w - 1 and w + 1
-  uint32_t not_first_w = 0, not_last_w = 0;
-#pragma unroll
-  for (int c = 0; c < RC; c++)
-#pragma unroll
-    for (int w = 0; w < WD; w++) {
-      if (w > 0) not_first_w |= 3u << (2 * (c * WD + w));
-      if (w + 1 < WD) not_last_w |= 3u << (2 * (c * WD + w));
-    }
-  for (int level = 0;; level++) {
-    const int rb = level & 1, wb = rb ^ 1;
-    uint32_t active;
-    {
-      const uint32_t gm = __builtin_amdgcn_readfirstlane(E.g[rb][wv]);
-      const uint32_t gu = __builtin_amdgcn_readfirstlane(
-          wv > 0 ? E.g[rb][wv > 0 ? wv - 1 : 0] : (RC > 1 ? E.g[rb][3] << (2 * WD) : 0u));
-      const uint32_t gd = __builtin_amdgcn_readfirstlane(
-          wv < 3 ? E.g[rb][wv < 3 ? wv + 1 : 3] : (RC > 1 ? E.g[rb][0] >> (2 * WD) : 0u));
-      active = gm | ((gm << 2) & not_first_w) | ((gm >> 2) & not_last_w) | gu | gd;
-    }
-    uint64_t N1[RC][WD], N2[RC][WD];
-#pragma unroll
-    for (int c = 0; c < RC; c++) {
-      // the row above lane 0 / below lane 63: the neighbouring wave of this
-      // chunk, or the last (first) wave of the previous (next) chunk
-      const int uc = wv > 0 ? c : c - 1, uw = wv > 0 ? wv - 1 : 3;
-      const int dc = wv < 3 ? c : c + 1, dw = wv < 3 ? wv + 1 : 0;
-      const bool has_u = uc >= 0, has_d = dc < RC;
-#pragma unroll
-      for (int w = 0; w < WD; w++) {
-        const int bit = 2 * (c * WD + w);
-        N1[c][w] = N2[c][w] = 0;
-        if ((active >> bit) & 1u) {
-          const uint64_t eu = has_u ? E.v[rb][has_u ? uc : 0][uw][1][0][w] : 0ull;
-          const uint64_t ed = has_d ? E.v[rb][has_d ? dc : 0][dw][0][0][w] : 0ull;
-          uint64_t h = (V1[c][w] << 1) | (V1[c][w] >> 1);
-          if (w > 0) h |= V1[c][w - 1] >> 63;
-          if (w + 1 < WD) h |= V1[c][w + 1] << 63;
-          N1[c][w] = (h | lane_up(V1[c][w], eu) | lane_down(V1[c][w], ed)) & M[c][w] & ~V1[c][w];
-        }
-        if ((active >> (bit + 1)) & 1u) {
-          const uint64_t eu = has_u ? E.v[rb][has_u ? uc : 0][uw][1][1][w] : 0ull;
-          const uint64_t ed = has_d ? E.v[rb][has_d ? dc : 0][dw][0][1][w] : 0ull;
-          uint64_t h = (V2[c][w] << 1) | (V2[c][w] >> 1);
-          if (w > 0) h |= V2[c][w - 1] >> 63;
-          if (w + 1 < WD) h |= V2[c][w + 1] << 63;
-          N2[c][w] = (h | lane_up(V2[c][w], eu) | lane_down(V2[c][w], ed)) & M[c][w] & ~V2[c][w];
-        }
-      }
-    }
+  for (int round = 0;; round++) {
     uint32_t grew = 0;
 #pragma unroll
-    for (int c = 0; c < RC; c++) {
+    for (int j = 0; j < kHalo; j++) {
 #pragma unroll
-      for (int w = 0; w < WD; w++) {
-        V1[c][w] |= N1[c][w];
-        NK[c][w] |= N2[c][w] & V1[c][w];
-        V2[c][w] |= N2[c][w];
-        const int bit = 2 * (c * WD + w);
-        if (__ballot(N1[c][w] != 0) != 0) grew |= 1u << bit;
-        if (__ballot(N2[c][w] != 0) != 0) grew |= 2u << bit;
-      }
-    }
-    if (lane == 0 || lane == 63) {
-      const int side = lane == 63;
-#pragma unroll
-      for (int c = 0; c < RC; c++) {
+      for (int c = 0; c < RB; c++) {
+        uint64_t N1[WD], N2[WD];
 #pragma unroll
         for (int w = 0; w < WD; w++) {
-          E.v[wb][c][wv][side][0][w] = V1[c][w];
-          E.v[wb][c][wv][side][1][w] = V2[c][w];
+          uint64_t h1 = (V1[c][w] << 1) | (V1[c][w] >> 1), h2 = (V2[c][w] << 1) | (V2[c][w] >> 1);
+          if (w > 0) {
+            h1 |= V1[c][w - 1] >> 63;
+            h2 |= V2[c][w - 1] >> 63;
+          }
+          if (w + 1 < WD) {
+            h1 |= V1[c][w + 1] << 63;
+            h2 |= V2[c][w + 1] << 63;
+          }
+          N1[w] = (h1 | lane_up(V1[c][w], 0ull) | lane_down(V1[c][w], 0ull)) & M[c][w] & ~V1[c][w];
+          N2[w] = (h2 | lane_up(V2[c][w], 0ull) | lane_down(V2[c][w], 0ull)) & M[c][w] & ~V2[c][w];
+        }
+#pragma unroll
+        for (int w = 0; w < WD; w++) {
+          V1[c][w] |= N1[w];
+          NK[c][w] |= N2[w] & V1[c][w];
+          V2[c][w] |= N2[w];
+          if (__ballot(own && (N1[w] | N2[w]) != 0) != 0) grew = 1;
         }
       }
+    }
+    // publish the outer own rows: lanes 4..7 (top) and 56..59 (bottom)
+    const int wb = round & 1;
+    const bool top = lane >= kHalo && lane < 2 * kHalo, bot = lane >= 64 - 2 * kHalo && own;
+    if (top || bot) {
+      const int side = bot, r = top ? lane - kHalo : lane - (64 - 2 * kHalo);
+#pragma unroll
+      for (int c = 0; c < RB; c++)
+#pragma unroll
+        for (int w = 0; w < WD; w++) {
+          E.v[wb][4 * c + wv][side][r][0][w] = V1[c][w];
+          E.v[wb][4 * c + wv][side][r][1][w] = V2[c][w];
+        }
     }
     if (lane == 0) E.g[wb][wv] = grew;
     __syncthreads();
-    const uint32_t any = E.g[wb][0] | E.g[wb][1] | E.g[wb][2] | E.g[wb][3];
-    if (__builtin_amdgcn_readfirstlane(any) == 0) break;
+    if (__builtin_amdgcn_readfirstlane(E.g[wb][0] | E.g[wb][1] | E.g[wb][2] | E.g[wb][3]) == 0)
+      break;
+    // lanes 0..3 take the band above's bottom rows, lanes 60..63 the band
+    // below's top rows (none above band 0 or below the last band: zeros)
+    const bool up = lane < kHalo, dn = lane >= 64 - kHalo;
+#pragma unroll
+    for (int c = 0; c < RB; c++) {
+      const int b = 4 * c + wv;
+      const int nb = up ? b - 1 : b + 1;
+      const bool ok = (up || dn) && nb >= 0 && nb < 4 * RB;
+      const int r = up ? lane : lane - (64 - kHalo);
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        const uint64_t e1 = ok ? E.v[wb][ok ? nb : 0][up ? 1 : 0][ok ? r : 0][0][w] : 0ull;
+        const uint64_t e2 = ok ? E.v[wb][ok ? nb : 0][up ? 1 : 0][ok ? r : 0][1][w] : 0ull;
+        V1[c][w] = (up || dn) ? e1 : V1[c][w];
+        V2[c][w] = (up || dn) ? e2 : V2[c][w];
+      }
+    }
   }
 #pragma unroll
-  for (int c = 0; c < RC; c++) {
-    const int nrows = min(64, bh - 256 * c - 64 * wv);
+  for (int c = 0; c < RB; c++) {
+    const int row0 = kOwn * (4 * c + wv) - kHalo;
 #pragma unroll
     for (int w = 0; w < WD; w++) {
       const uint64_t K = V1[c][w] & ~NK[c][w];
       const int col = 64 * w + lane;
-      for (int rr = 0; rr < nrows; rr++) {
+      for (int rr = kHalo; rr < 64 - kHalo; rr++) {
+        const int y = row0 + rr;
+        if (y >= bh) break;  // wave-uniform
         const uint64_t word = __shfl(K, rr);
-        if (col < bw && ((word >> lane) & 1ull))
-          halves[(by0 + 256 * c + 64 * wv + rr) * W + bx0 + col] = k2;
+        if (col < bw && ((word >> lane) & 1ull)) halves[(by0 + y) * W + bx0 + col] = k2;
       }
     }
   }
@@ -1035,7 +1012,7 @@ __device__ __forceinline__ void bfs_rows(const int* __restrict__ seg, int W, int
 
 __global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direction) {
   __shared__ uint64_t sM[kBlkWords], sV1[kBlkWords], sV2[kBlkWords], sNK[kBlkWords];
-  __shared__ RowsEdge edge;
+  __shared__ BandEdge edge;
   __shared__ int item, changed[3];
   const int tid = threadIdx.x;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
@@ -1060,27 +1037,23 @@ __global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direct
     const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
     int* halves = F.halves;
     if (s1 < 0) continue;  // no pixel has d1 > d2: the label keeps all its pixels
-#define STSP_ROWS(RC, WD) \
-  bfs_rows<RC, WD>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, edge)
-    // (4 chunks of 6 words would not fit in registers)
-    if (bh <= 256 * kRowsChunks && bw <= 64 * kRowsWords && (bh <= 512 || bw <= 256)) {
-      const int rc = bh <= 256 ? 1 : (bh <= 512 ? 2 : 4);
+#define STSP_BANDS(RB, WD) \
+  bfs_bands<RB, WD>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, edge)
+    if (bh <= 4 * kOwn * kBandChunks && bw <= 64 * kBandWords) {
+      const int rb = bh <= 4 * kOwn ? 1 : 2;
       const int wd = bw <= 128 ? 2 : (bw <= 256 ? 4 : 6);
-      if (rc == 1) {
-        if (wd == 2) STSP_ROWS(1, 2);
-        else if (wd == 4) STSP_ROWS(1, 4);
-        else STSP_ROWS(1, 6);
-      } else if (rc == 2) {
-        if (wd == 2) STSP_ROWS(2, 2);
-        else if (wd == 4) STSP_ROWS(2, 4);
-        else STSP_ROWS(2, 6);
+      if (rb == 1) {
+        if (wd == 2) STSP_BANDS(1, 2);
+        else if (wd == 4) STSP_BANDS(1, 4);
+        else STSP_BANDS(1, 6);
       } else {
-        if (wd == 2) STSP_ROWS(4, 2);
-        else STSP_ROWS(4, 4);
+        if (wd == 2) STSP_BANDS(2, 2);
+        else if (wd == 4) STSP_BANDS(2, 4);
+        else STSP_BANDS(2, 6);
       }
       continue;
     }
-#undef STSP_ROWS
+#undef STSP_BANDS
     const int WD = (bw + 63) >> 6;
     const int nw = bh * WD;
     if (nw <= kBlkWords) {

@@ -36,6 +36,20 @@ __device__ __forceinline__ int pro_local(const Pro& p, int kind, int f, int b) {
   const ProFrame& F = p.fr[f];                               \
   const int k = pro_local(p, kProLab, f, blockIdx.x) * kBlock + threadIdx.x;
 
+// adds the block's count of true predicates to *dst with one atomic (all
+// threads of the block must call it)
+__device__ __forceinline__ void block_count_add(bool pred, int* dst) {
+  __shared__ int wc[kBlock / 64];
+  const unsigned long long b = __ballot(pred);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int w = 0; w < kBlock / 64; w++) n += wc[w];
+    if (n) atomicAdd(dst, n);
+  }
+}
+
 int pro_blocks(int kind, int W, int H, int np, int nbuf) {
   switch (kind) {
     case kProPix: return div_up((long long)W * H, kBlock);
@@ -153,9 +167,7 @@ __global__ void kp_count_labels(Pro p, int which) {
 // holes of the shifted map (the adaptive iteration count, pyapi.cu:139-149)
 __global__ void kp_count_holes(Pro p) {
   PRO_PIX
-  const bool neg = t < F.npix && F.shifted[t] < 0;
-  const unsigned long long b = __ballot(neg);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(p.negs + f, __popcll(b));
+  block_count_add(t < F.npix && F.shifted[t] < 0, p.negs + f);
 }
 
 void pro_shift(const Pro& p, const int* total, hipStream_t s) {
@@ -193,8 +205,7 @@ __global__ void kp_fill_border(Pro p, int r) {
     F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
     neg = seg[t] < 0;
   }
-  unsigned long long b = __ballot(neg);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(p.fillcnt + r * p.nf + f, __popcll(b));
+  block_count_add(neg, p.fillcnt + r * p.nf + f);
 }
 
 __global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {

@@ -79,11 +79,11 @@ def test_bass_parity_config0(orc):
 
 @pytest.mark.parametrize("H,W,sp,niters,nreg,seed", [
     (200, 260, 60, 12, 6, 1),    # boxes over 64 px: two-word / two-chunk wave BFS
-    (300, 420, 140, 12, 5, 2),   # boxes over 128 px early on: workgroup row BFS
-    (250, 820, 330, 8, 3, 7),    # boxes 129-256 px: row BFS with 2 and 4 words per row
-    (200, 1100, 420, 6, 2, 5),   # boxes 257-384 px wide: row BFS with 6 words per row
-    (1100, 200, 600, 6, 2, 9),   # boxes over 512 rows: row BFS with four row chunks
-    (700, 900, 500, 8, 3, 3)])   # two-chunk row BFS, LDS bitsets and global-memory BFS
+    (300, 420, 140, 12, 5, 2),   # boxes over 128 px early on: workgroup band BFS
+    (250, 820, 330, 8, 3, 7),    # boxes 129-256 px: band BFS with 2 and 4 words per row
+    (200, 1100, 420, 6, 2, 5),   # boxes 257-384 px wide: band BFS with 6 words per row
+    (1100, 200, 600, 6, 2, 9),   # boxes over 448 rows: LDS bitset BFS
+    (700, 900, 500, 8, 3, 3)])   # two-chunk band BFS, LDS bitsets and global-memory BFS
 def test_split_bfs_box_sizes(orc, H, W, sp, niters, nreg, seed):
     """Large superpixels route the split proposal through every BFS path."""
     img = synth.voronoi_image(H, W, nreg, seed=seed)
