@@ -27,6 +27,7 @@ METRIC = "frames/s + p99 frame latency, 16 mixed-res streams, served vs plain lo
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PX_SWEEP = 20  # labels r 4 + img r 12 + labels w 4 per sweep
 SWEEPS_PER_ITER = 4      # niters_seg
+SWEEP_TIMING_LAUNCHES = 50  # back-to-back sweep launches timed for the roofline
 
 
 def parse():
@@ -145,21 +146,13 @@ def run_served(args, streams, torch, core):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     lat = [r.done - r.arrival for r in srv.completed]
-    # dominant kernel: the batched fused sweep (all 16 frames per launch),
-    # timed with HIP events on an extra untimed step
-    lead = srv.streams[sids[0]].ctx
-    for sid in sids:
-        srv.streams[sid].ctx.enable_timing(True)
-    t = args.warmup + args.steps
-    for sid, s in zip(sids, streams):
-        srv.submit(sid, s["vid"][t], s["flows"][t - 1])
-    while srv.pending():
-        srv.step()
-    torch.cuda.synchronize()
-    ms, groups = lead.kernel_time(0)  # one event pair per 4-sweep group
+    # dominant kernel: the batched fused sweep (all 16 frames per launch) of
+    # the last step's batch, launched back to back between two HIP events on
+    # the serving stream (no event between launches, so no dispatch bubbles)
+    launches = SWEEP_TIMING_LAUNCHES
+    ms = srv.runner.time_sweeps(launches, stream=srv.cuda_stream)
     px = sum(s["H"] * s["W"] for s in streams)
     per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
-    launches = groups * (SWEEPS_PER_ITER // per_launch)
     srv.close()
     return {"elapsed": elapsed, "lat": lat, "sweep_ms": ms,
             "sweep_bytes": launches * BYTES_PER_PX_SWEEP * per_launch * px,

@@ -162,6 +162,16 @@ enum { STSP_KCLASS_SWEEP = 0, STSP_KCLASS_STATS = 1, STSP_KCLASS_COUNT = 2 };
 int stsp_ctx_enable_timing(stsp_ctx* ctx, int on);
 int stsp_ctx_kernel_time(stsp_ctx* ctx, int kclass, double* total_ms, long long* launches);
 
+/* Times the fused boundary sweep alone: after one (untimed) recount of the
+ * label statistics and sweep records from the frames' final labels,
+ * `nlaunch` launches back to back over the frames of the batch's last
+ * refinement, between two hipEvents on `stream`; *ms_out receives the
+ * elapsed time.  Each launch reads the final labels and writes the second
+ * label buffer, so results already returned are unchanged; the working
+ * statistics and parameters are rebuilt by the next frame's prologue.  For
+ * benchmarks (no counterpart in the reference). */
+int stsp_batch_time_sweeps(stsp_batch* batch, int nlaunch, hipStream_t stream, float* ms_out);
+
 const char* stsp_status_string(int status);
 
 #ifdef __cplusplus

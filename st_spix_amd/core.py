@@ -240,6 +240,19 @@ class BatchRunner:
         except Exception:
             pass
 
+    def time_sweeps(self, nlaunch: int, stream=None) -> float:
+        """Milliseconds of `nlaunch` back-to-back launches of the fused
+        boundary sweep over the frames of the last batch (HIP events on the
+        launch stream; stsp_batch_time_sweeps).  For benchmarks."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        ms = ctypes.c_float()
+        _lib.check(_lib.lib().stsp_batch_time_sweeps(self.handle, int(nlaunch),
+                                                     ctypes.c_void_p(s.cuda_stream),
+                                                     ctypes.byref(ms)),
+                   "stsp_batch_time_sweeps")
+        return float(ms.value)
+
     def bass(self, items, stream=None):
         """items: list of (ctx, img).  First frames (BASS) of several images or
         streams in shared launches.  Returns a list of (spix, params)."""

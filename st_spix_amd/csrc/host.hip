@@ -126,6 +126,9 @@ struct stsp_batch {
   std::vector<long long> gkey;
   int gcur = 0;
   Lanes lanes;
+  // the last refinement's launch settings (for stsp_batch_time_sweeps)
+  SimpleLut last_lut = {};
+  int last_shape = -1;
   // propagation prologue: per-frame table, block offsets, per-round hole
   // counts ([kFillMax][nf] on the device, one chunk of rounds on the host)
   ProFrame* d_pro = nullptr;
@@ -442,8 +445,10 @@ static void tstart(stsp_ctx* c, int cls, hipStream_t s) {
   KTimer& t = c->timers[cls];
   if (t.n >= KTimer::kMax) return;
   if (t.created <= t.n) {
-    (void)hipEventCreate(&t.ev[2 * t.n]);
-    (void)hipEventCreate(&t.ev[2 * t.n + 1]);
+    // timing-only events: no system-scope fence (cache write-back and
+    // invalidation) at record time, which would be counted in the interval
+    (void)hipEventCreateWithFlags(&t.ev[2 * t.n], hipEventDisableSystemFence);
+    (void)hipEventCreateWithFlags(&t.ev[2 * t.n + 1], hipEventDisableSystemFence);
     t.created = t.n + 1;
   }
   (void)hipEventRecord(t.ev[2 * t.n], s);
@@ -738,6 +743,8 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   }
   int rc = batch_build(bh, ctxs, n, s);
   if (rc) return rc;
+  bh->last_lut = lead->lut;
+  bh->last_shape = shape;
   int cur;
   bool timing = lead->timing != 0;
   // The legacy default stream cannot be captured: there the launches run
@@ -1152,6 +1159,26 @@ void stsp_batch_destroy(stsp_batch* bh) {
   (void)hipDeviceSynchronize();
   batch_free(bh);
   delete bh;
+}
+
+int stsp_batch_time_sweeps(stsp_batch* bh, int nlaunch, hipStream_t s, float* ms_out) {
+  if (!bh || !ms_out || nlaunch < 1 || bh->last_shape < 0) return STSP_ERR_ARG;
+  hipEvent_t e[2];
+  HIP_TRY(hipEventCreateWithFlags(&e[0], hipEventDisableSystemFence));
+  HIP_TRY(hipEventCreateWithFlags(&e[1], hipEventDisableSystemFence));
+  // compaction renumbered the labels after the last refinement: recount the
+  // statistics and sweep records so the timed sweeps see a consistent state
+  b_update_params(bh->b, 0, false, s);
+  const int per = kSweepShape[bh->last_shape].s;
+  HIP_TRY(hipEventRecord(e[0], s));
+  for (int i = 0; i < nlaunch; i++) (void)b_sweeps(bh->b, 0, per, bh->last_shape, bh->last_lut, s);
+  HIP_TRY(hipEventRecord(e[1], s));
+  HIP_TRY(hipEventSynchronize(e[1]));
+  const hipError_t er = hipEventElapsedTime(ms_out, e[0], e[1]);
+  (void)hipEventDestroy(e[0]);
+  (void)hipEventDestroy(e[1]);
+  HIP_TRY(hipGetLastError());
+  return er == hipSuccess ? STSP_OK : STSP_ERR_HIP;
 }
 
 int stsp_batch_use_graph(stsp_batch* bh, int on) {
