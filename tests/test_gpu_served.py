@@ -154,3 +154,40 @@ def test_bass_batch_config2_sizes(orc):
     runner.close()
     for c in ctxs:
         c.close()
+
+
+def test_time_sweeps_leaves_results_and_next_frame_unchanged(orc):
+    """stsp_batch_time_sweeps (the benchmark hook) times extra sweep
+    launches over the last batch: the results already returned stay as they
+    are and the next frame of every stream still equals the oracle's."""
+    T = 3
+    kw = dict(niters=12, potts=10.0, prop_niters=10)
+    vids = make_videos(T)
+    ref = run_oracle(orc, vids, kw)
+    ctxs, state = [], []
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        for (H, W, sp, _), (vid, flows) in zip(STREAMS, vids):
+            cfg = core.make_config(sp_size=sp, **kw)
+            ctx = core.Context(H, W, cfg)
+            spix, params = core.bass_frame(cuda(vid[0]), cfg, ctx=ctx)
+            ctxs.append(ctx)
+            state.append((spix, params))
+        runner = core.BatchRunner(8, use_graph=True)
+        for t in range(1, T):
+            items = [(ctxs[i], cuda(vids[i][0][t]), cuda(vids[i][1][t - 1]), state[i][0],
+                      state[i][1]) for i in range(len(STREAMS))]
+            outs = runner.prop(items, stream=side)
+            side.synchronize()
+            before = [o[0].clone() for o in outs]
+            ms = runner.time_sweeps(6, stream=side)
+            side.synchronize()
+            assert ms > 0.0
+            for i, (spix, params, _) in enumerate(outs):
+                assert torch.equal(spix, before[i]), f"stream{i} frame{t} changed"
+                assert_labels(spix, ref[i][t][0], f"stream{i} frame{t}")
+                assert_params(params, ref[i][t][1], f"stream{i} frame{t}")
+                state[i] = (spix, params)
+    runner.close()
+    for c in ctxs:
+        c.close()
