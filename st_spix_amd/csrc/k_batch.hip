@@ -832,15 +832,14 @@ __device__ __forceinline__ void bfs_wave(const int* __restrict__ seg, int W, int
   }
 }
 
-// Persistent waves over the wave list.  Loop control is wave-uniform
-// (readfirstlane) and free of lane-conditional branches.
+// Persistent waves over the wave list, wave w taking items w, w + nwaves,
+// ... (no shared cursor: thousands of waves would contend on one atomic).
+// Loop control is wave-uniform and free of lane-conditional branches.
 __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int direction) {
-  const int lane = threadIdx.x & 63;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[0], b.work_cap));
-  while (true) {
-    const int mine = atomicAdd(b.work_n + 1, lane == 0 ? 1 : 0);
-    const int it = __builtin_amdgcn_readfirstlane(mine);
-    if (it >= n_items) break;
+  const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int nwaves = gridDim.x * 4;
+  for (int it = gw; it < n_items; it += nwaves) {
     const int wv = __builtin_amdgcn_readfirstlane(b.work[it]);
     const int f = wv >> 24, k = wv & 0xffffff;
     const FrameDesc& F = b.fr[f];
@@ -1504,10 +1503,9 @@ __global__ void kb_relabel_lists(Batch b) {
 __global__ void __launch_bounds__(256) kb_relabel_search(Batch b) {
   const int lane = threadIdx.x & 63;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[0], b.work_cap));
-  while (true) {
-    const int mine = atomicAdd(b.work_n + 1, lane == 0 ? 1 : 0);
-    const int it = __builtin_amdgcn_readfirstlane(mine);
-    if (it >= n_items) break;
+  const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int nwaves = gridDim.x * 4;
+  for (int it = gw; it < n_items; it += nwaves) {
     const int wv = b.work[it];
     const int f = wv >> 24, s = wv & 0xffffff;
     const FrameDesc& F = b.fr[f];
