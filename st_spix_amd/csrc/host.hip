@@ -612,10 +612,12 @@ static inline bool is_merge_iter(const stsp_config& cfg, int idx) {
 static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& side, int idx,
                            int prop_mode, int shape, int* cur, int* smc, hipStream_t s) {
   const stsp_config& cfg = lead->cfg;
+  // split and merge leave the label statistics current (kb_split_stats,
+  // kb_merge_stats): only the parameters are updated after them
   if (is_split_iter(cfg, idx)) {
     *smc += 1;
     b_split(b, *cur, *smc % 2 + 1, s, side);
-    b_update_params(b, *cur, false, s);
+    b_param_only(b, s);
   }
   if (is_merge_iter(cfg, idx)) {
     if (prop_mode) {
@@ -623,7 +625,7 @@ static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& sid
       *smc += 1;
     }
     b_merge(b, *cur, *smc % 2, s);
-    b_update_params(b, *cur, false, s);
+    b_param_only(b, s);
   }
   tstart(lead, STSP_KCLASS_SWEEP, s);
   *cur = b_sweeps(b, *cur, cfg.niters_seg, shape, lead->lut, s);

@@ -1279,6 +1279,38 @@ __global__ void kb_split_apply(Batch b, int cur) {
   }
 }
 
+// Label statistics after a split without a recount: the halves pass
+// gathered both parts of every label (k and k + max_spix + 1) with exact
+// integer sums, and the labels that did not split kept their pixels, so an
+// accepted label takes the sums of the part that stays and its new label
+// those of the part that moves (the assignment of kb_split_apply).  Must run
+// before kb_add_max changes max_spix.  (The live-label colour squares and
+// the invalid counts are not kept for live labels.)
+__global__ void kb_split_stats(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const int ms = *F.max_spix;
+  if (ms > F.nbuf / 2 || !F.sm.accept[k]) return;
+  const int s = F.sm.pairs[2 * k];
+  const int k2 = k + (ms + 1);
+  const bool move_k2 = F.prop_mode ? (F.sm.select[k] != 0) : true;
+  const int stay = move_k2 ? k : k2, go = move_k2 ? k2 : k;
+  const LabelStats& st = F.st;
+  const LabelStats& h = F.sst;
+  st.count[k] = h.count[stay];
+  st.count[s] = h.count[go];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    st.S[3 * k + c] = h.S[3 * stay + c];
+    st.S[3 * s + c] = h.S[3 * go + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+    st.X[5 * k + c] = h.X[5 * stay + c];
+    st.X[5 * s + c] = h.X[5 * go + c];
+  }
+}
+
 __global__ void kb_add_max(Batch b) {
   int f = threadIdx.x;
   if (f >= b.nf) return;
@@ -1306,6 +1338,7 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
   kb_split_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_split_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_split_stats<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_add_max<<<1, 64, 0, s>>>(b);
 }
 
@@ -1448,6 +1481,26 @@ __global__ void kb_merge_apply(Batch b, int cur) {
   if (k >= 0 && k < F.nbuf && F.sm.remove[k]) seg[t] = F.sm.pairs[2 * k + 1];
 }
 
+// Label statistics after a merge without a recount: the merge's own pass
+// gathered every live label's exact integer sums (into the split/merge
+// slots), and kb_merge_apply moves the pixels of a removed label k to
+// pairs[2k + 1] in one hop, so each label's sums are added to its
+// destination (after kb_stats_clear).
+__global__ void kb_merge_stats(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const LabelStats& h = F.sst;
+  const int n = h.count[k];
+  if (n == 0) return;
+  const int x = F.sm.remove[k] ? F.sm.pairs[2 * k + 1] : k;
+  const LabelStats& st = F.st;
+  atomicAdd(st.count + x, n);
+#pragma unroll
+  for (int c = 0; c < 3; c++) atomic_add_i64(st.S + 3 * x + c, h.S[3 * k + c]);
+#pragma unroll
+  for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * x + c, h.X[5 * k + c]);
+}
+
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
@@ -1459,6 +1512,8 @@ void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_merge_flag<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_remove<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
+  kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
+  kb_merge_stats<<<b.lab_blocks, kBlock, 0, s>>>(b);
 }
 
 // ---------------------------------------------------------------------------
