@@ -114,11 +114,15 @@ def get_context(height: int, width: int, cfg: _lib.Config, device) -> Context:
     return ctx
 
 
-def alloc_params(cap: int, device) -> Params:
+def alloc_params(cap: int, device, zero: bool = True) -> Params:
+    """Parameter arrays of `cap` slots.  The export writes every field of
+    slots [0, nspix) and results are trimmed to nspix, so callers on the
+    served path skip the zero fill (zero=False)."""
     out = {}
+    make = torch.zeros if zero else torch.empty
     for name, dt, w in _lib.PARAM_FIELDS:
         shape = (cap, w) if w > 1 else (cap,)
-        out[name] = torch.zeros(shape, dtype=_DTYPES[dt], device=device)
+        out[name] = make(shape, dtype=_DTYPES[dt], device=device)
     return out
 
 
@@ -273,7 +277,7 @@ class BatchRunner:
             if tuple(img.shape[:2]) != (ctx.H, ctx.W):
                 raise RuntimeError("img size does not match its context")
             cap = _lib.lib().stsp_bass_capacity(ctx.cfg.sp_size, ctx.H, ctx.W)
-            out = alloc_params(cap, img.device)
+            out = alloc_params(cap, img.device, zero=False)
             spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
             ctxs[i] = ctx.handle.value
             imgs[i] = img.data_ptr()
@@ -311,7 +315,7 @@ class BatchRunner:
                 _check_input(t, name)
             np_ = int(params_prev["ids"].shape[0])
             cap = _lib.lib().stsp_prop_capacity(np_)
-            out = alloc_params(cap, img.device)
+            out = alloc_params(cap, img.device, zero=False)
             spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
             ctxs[i] = ctx.handle.value
             imgs[i] = img.data_ptr()

@@ -139,6 +139,8 @@ struct stsp_batch {
   int* h_pro_off = nullptr;
   int* h_fillcnt = nullptr;
   int* h_negs = nullptr;
+  int* d_live = nullptr;  // frames still filling: offsets [kProKinds][kProMax + 1], then map
+  int* h_live = nullptr;
 };
 
 struct stsp_ctx {
@@ -342,6 +344,9 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipHostMalloc((void**)&bh->h_fillcnt, sizeof(int) * kFillChunk * maxf,
                         hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf, hipHostMallocDefault));
+  const size_t live = sizeof(int) * (kProKinds * (kProMax + 1) + kProMax);
+  HIP_TRY(hipMalloc((void**)&bh->d_live, live));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_live, live, hipHostMallocDefault));
   return STSP_OK;
 }
 
@@ -422,6 +427,9 @@ static void batch_free(stsp_batch* bh) {
   if (bh->h_pro_off) (void)hipHostFree(bh->h_pro_off);
   if (bh->h_fillcnt) (void)hipHostFree(bh->h_fillcnt);
   if (bh->h_negs) (void)hipHostFree(bh->h_negs);
+  if (bh->d_live) (void)hipFree(bh->d_live);
+  if (bh->h_live) (void)hipHostFree(bh->h_live);
+  bh->d_live = bh->h_live = nullptr;
   bh->d_pro = nullptr;
   bh->d_pro_off = nullptr;
   bh->d_fillcnt = bh->d_negs = nullptr;
@@ -861,10 +869,14 @@ static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, Lanes
 static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStream_t s) {
   const int n = p.nf;
   std::vector<int> prev(n, -1), live(n, 1);
+  int nlive = n;
+  Pro q = p;  // the launches: every frame, later only those still filling
+  int qtotal[kProKinds];
+  for (int k = 0; k < kProKinds; k++) qtotal[k] = total[k];
   int chunk = 4;  // most frames finish within a few rounds
   for (int base = 0; base < kFillMax; base += chunk, chunk = kFillChunk) {
     const int rounds = kFillMax - base < chunk ? kFillMax - base : chunk;
-    pro_fill_rounds(p, total, base, rounds, s);
+    pro_fill_rounds(q, qtotal, base, rounds, s);
     HIP_TRY(hipMemcpyAsync(bh->h_fillcnt, p.fillcnt + base * n, sizeof(int) * rounds * n,
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -883,6 +895,31 @@ static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStre
       any |= live[i];
     }
     if (!any) return STSP_OK;
+    int m = 0;
+    for (int i = 0; i < n; i++) m += live[i];
+    if (m < nlive) {  // launch over the frames still filling from now on
+      nlive = m;
+      int* off = bh->h_live;
+      int* fmap = bh->h_live + kProKinds * (kProMax + 1);
+      const int* all = bh->h_pro_off;
+      for (int k = 0; k < kProKinds; k++) off[k * (kProMax + 1)] = 0;
+      for (int i = 0, j = 0; i < n; i++) {
+        if (!live[i]) continue;
+        fmap[j] = i;
+        for (int k = 0; k < kProKinds; k++)
+          off[k * (kProMax + 1) + j + 1] = off[k * (kProMax + 1) + j] +
+                                           all[k * (kProMax + 1) + i + 1] -
+                                           all[k * (kProMax + 1) + i];
+        j++;
+      }
+      for (int k = 0; k < kProKinds; k++) qtotal[k] = off[k * (kProMax + 1) + m];
+      HIP_TRY(hipMemcpyAsync(bh->d_live, bh->h_live,
+                             sizeof(int) * (kProKinds * (kProMax + 1) + kProMax),
+                             hipMemcpyHostToDevice, s));
+      q.off = bh->d_live;
+      q.fmap = bh->d_live + kProKinds * (kProMax + 1);
+      q.nf = m;
+    }
   }
   return STSP_OK;
 }
@@ -945,6 +982,8 @@ static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* co
   p.fr = bh->d_pro;
   p.off = bh->d_pro_off;
   p.nf = n;
+  p.fmap = nullptr;
+  p.nall = n;
   p.fillcnt = bh->d_fillcnt;
   p.negs = bh->d_negs;
   pro_shift(p, total, s);

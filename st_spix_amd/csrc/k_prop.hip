@@ -187,14 +187,19 @@ void pro_shift(const Pro& p, const int* total, hipStream_t s) {
 // fills border holes in four checkerboard phases.  A frame whose count was
 // zero in the round before is finished: its blocks return at once.
 // ---------------------------------------------------------------------------
+// The filling rounds are launched over the frames that still have holes
+// (p.fmap lists them; the counts keep the table's frame index).
 __global__ void kp_fill_zero(Pro p, int base, int rounds) {
-  const int n = rounds * p.nf;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nf + i] = 0;
+  const int n = rounds * p.nall;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nall + i] = 0;
 }
 
 __global__ void kp_fill_border(Pro p, int r) {
-  PRO_PIX
-  if (r > 0 && p.fillcnt[(r - 1) * p.nf + f] == 0) return;  // block-uniform
+  const int fl = pro_frame(p, kProPix, blockIdx.x);
+  const int f = p.fmap ? p.fmap[fl] : fl;
+  const ProFrame& F = p.fr[f];
+  const int t = pro_local(p, kProPix, fl, blockIdx.x) * kBlock + threadIdx.x;
+  if (r > 0 && p.fillcnt[(r - 1) * p.nall + f] == 0) return;  // block-uniform
   bool neg = false;
   if (t < F.npix) {
     const int W = F.W, H = F.H;
@@ -205,16 +210,17 @@ __global__ void kp_fill_border(Pro p, int r) {
     F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
     neg = seg[t] < 0;
   }
-  block_count_add(neg, p.fillcnt + r * p.nf + f);
+  block_count_add(neg, p.fillcnt + r * p.nall + f);
 }
 
 __global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {
-  const int f = pro_frame(p, kProHalf, blockIdx.x);
-  if (p.fillcnt[r * p.nf + f] == 0) return;
+  const int fl = pro_frame(p, kProHalf, blockIdx.x);
+  const int f = p.fmap ? p.fmap[fl] : fl;
+  if (p.fillcnt[r * p.nall + f] == 0) return;
   const ProFrame& F = p.fr[f];
   const int W = F.W, H = F.H;
   int hw = (W - xm + 1) / 2, hh = (H - ym + 1) / 2;
-  int i = pro_local(p, kProHalf, f, blockIdx.x) * kBlock + threadIdx.x;
+  int i = pro_local(p, kProHalf, fl, blockIdx.x) * kBlock + threadIdx.x;
   if (i >= hw * hh) return;
   int* seg = F.seg;
   int x = 2 * (i % hw) + xm, y = 2 * (i / hw) + ym;

@@ -75,10 +75,12 @@ struct ProFrame {
 enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProKinds };
 struct Pro {
   const ProFrame* fr;  // device table
-  const int* off;      // [kProKinds][kProMax + 1]
-  int nf;
-  int* fillcnt;        // [kFillMax][nf]: holes at the start of each round
-  int* negs;           // [nf]: holes of the shifted map
+  const int* off;      // [kProKinds][kProMax + 1] over the frames launched
+  int nf;              // frames launched (entries of off)
+  const int* fmap;     // launched frame i is table frame fmap[i] (null: i itself)
+  int nall;            // frames in the table
+  int* fillcnt;        // [kFillMax][nall]: holes at the start of each round
+  int* negs;           // [nall]: holes of the shifted map
 };
 // host side: block counts per kind for a frame
 int pro_blocks(int kind, int W, int H, int np, int nbuf);
