@@ -1043,6 +1043,26 @@ static int epilogue_finish(stsp_ctx* c, int* spix_out, stsp_params* out, int* ns
   return STSP_OK;
 }
 
+// The batched form of the export: the superpixel count stays on the device
+// (total = nspix_prev + the compaction's new-label count), so no host wait
+// sits between compaction and export; the count and the capacity check are
+// read once after the whole batch.
+__global__ void k_total_from(int* total, const int* num_new, int prev) {
+  *total = prev + *num_new;
+}
+static int epilogue_finish_async(stsp_ctx* c, int* spix_out, const stsp_params* out,
+                                 hipStream_t s) {
+  k_total_from<<<1, 1, 0, s>>>(c->total, c->num_new, c->nspix_prev);
+  launch_fill(c->alive, 0, c->nbuf, s);
+  launch_mark_alive(c->seg, c->npix, c->alive, c->nbuf, s);
+  OutParams o;
+  memcpy(&o, out, sizeof(o));
+  launch_export(c->P, o, c->total, c->alive, out->cap, s);
+  HIP_TRY(hipMemcpyAsync(spix_out, c->seg, sizeof(int) * c->npix, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
 static int run_single(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_out,
                       hipStream_t s) {
   stsp_ctx* one = c;
@@ -1259,17 +1279,18 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
     rc = epilogue_compact(ctxs[i], lane_of(L, s, i));
+    if (!rc) rc = epilogue_finish_async(ctxs[i], spix_out[i], &outs[i], lane_of(L, s, i));
     if (rc) return rc;
   }
-  rc = lanes_sync(L, s);
+  rc = lanes_join(L, s);
   if (rc) return rc;
-  if (L->n) HIP_TRY(hipStreamSynchronize(s));  // the refinement's timing events
+  HIP_TRY(hipStreamSynchronize(s));  // the counts (and the refinement's timing events)
   for (int i = 0; i < n; i++) tflush(ctxs[i]);  // timed launches are recorded on group leads
   for (int i = 0; i < n; i++) {
-    rc = epilogue_finish(ctxs[i], spix_out[i], &outs[i], &nspix_out[i], lane_of(L, s, i));
-    if (rc) return rc;
+    nspix_out[i] = ctxs[i]->nspix_prev + ctxs[i]->host[1];
+    if (nspix_out[i] > outs[i].cap) return STSP_ERR_CAPACITY;
   }
-  return lanes_join(L, s);
+  return STSP_OK;
 }
 
 static int check_batch_ctxs(stsp_batch* bh, stsp_ctx** ctxs, int n) {
