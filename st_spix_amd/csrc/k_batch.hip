@@ -113,6 +113,9 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
     if (x < W && y < H) {
       const int t = y * W + x;
       const int v = labs[t];
+      // the pass before a split also starts the halves map (a copy of the
+      // labels; the split search then writes the moving parts)
+      if (BBOX) F.halves[t] = v;
       if (v >= 0 && v < nlab) {
         k[j] = v;
         pend |= 1u << j;
@@ -685,10 +688,6 @@ __global__ void kb_sm_clear(Batch b) {
   }
 }
 
-__global__ void kb_copy_halves(Batch b, int cur) {
-  PIX_PROLOGUE
-  if (t < F.npix) F.halves[t] = F.seg[cur][t];
-}
 
 // Labels that take part in the split (valid, non-empty, split not skipped),
 // sorted by the size of their bounding box: boxes up to kWaveBox x kWaveBox
@@ -1322,9 +1321,10 @@ __global__ void kb_add_max(Batch b) {
 
 // The two BFS kernels write disjoint pixels (different labels): the few
 // long workgroup searches run on the side stream beside the wave searches.
+// Expects the halves map to equal the labels: the statistics pass with
+// bounding boxes that precedes every split writes it.
 void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
-  kb_copy_halves<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_reset_work<<<1, 64, 0, s>>>(b);
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
   (void)hipEventRecord(side.fork, s);
