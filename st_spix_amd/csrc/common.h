@@ -35,8 +35,9 @@ struct alignas(16) SweepRec {
 };
 
 // Per-label sufficient statistics.  S = colour sums, Q = colour square sums
-// (2^-32 fixed point), positions exact.  Q is gathered only for the split and
-// merge statistics; nothing reads it for the live labels.
+// (2^-32 fixed point), positions exact.  The live labels' sums (Q included,
+// which the merge reads) are kept exact between full passes by the sweeps'
+// per-pixel moves and the split/merge combinations.
 struct LabelStats {
   int* count;    // [cap]
   int64_t* S;    // [cap*3]

@@ -332,9 +332,9 @@ void b_param_only(const Batch& b, hipStream_t s) {
 void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, bbox ? 1 : 0);
   if (bbox)
-    kb_stats_accum<true, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<true, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   else
-    kb_stats_accum<false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
@@ -562,15 +562,18 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     const int ol = seg_in[g];
     if (nl == ol) continue;
     const int64_t x = gx, y = gy;
-    // (the live-label colour squares are never read, so they are not kept)
+    // (the colour squares move too: the merge reads them)
     const int64_t v[8] = {fx_color(img[3 * g]), fx_color(img[3 * g + 1]), fx_color(img[3 * g + 2]),
                           x, y, x * x, x * y, y * y};
     atomicAdd(st.count + ol, -1);
     atomicAdd(st.count + nl, 1);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
+      const int64_t q = fx_square(img[3 * g + c]);
       atomic_add_i64(st.S + 3 * ol + c, -v[c]);
       atomic_add_i64(st.S + 3 * nl + c, v[c]);
+      atomic_add_i64(st.Q + 3 * ol + c, -q);
+      atomic_add_i64(st.Q + 3 * nl + c, q);
     }
 #pragma unroll
     for (int c = 0; c < 5; c++) {
@@ -1283,8 +1286,8 @@ __global__ void kb_split_apply(Batch b, int cur) {
 // integer sums, and the labels that did not split kept their pixels, so an
 // accepted label takes the sums of the part that stays and its new label
 // those of the part that moves (the assignment of kb_split_apply).  Must run
-// before kb_add_max changes max_spix.  (The live-label colour squares and
-// the invalid counts are not kept for live labels.)
+// before kb_add_max changes max_spix.  (The invalid counts are not kept for
+// live labels.)
 __global__ void kb_split_stats(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
@@ -1302,6 +1305,8 @@ __global__ void kb_split_stats(Batch b) {
   for (int c = 0; c < 3; c++) {
     st.S[3 * k + c] = h.S[3 * stay + c];
     st.S[3 * s + c] = h.S[3 * go + c];
+    st.Q[3 * k + c] = h.Q[3 * stay + c];
+    st.Q[3 * s + c] = h.Q[3 * go + c];
   }
 #pragma unroll
   for (int c = 0; c < 5; c++) {
@@ -1496,16 +1501,41 @@ __global__ void kb_merge_stats(Batch b) {
   const LabelStats& st = F.st;
   atomicAdd(st.count + x, n);
 #pragma unroll
-  for (int c = 0; c < 3; c++) atomic_add_i64(st.S + 3 * x + c, h.S[3 * k + c]);
+  for (int c = 0; c < 3; c++) {
+    atomic_add_i64(st.S + 3 * x + c, h.S[3 * k + c]);
+    atomic_add_i64(st.Q + 3 * x + c, h.Q[3 * k + c]);
+  }
 #pragma unroll
   for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * x + c, h.X[5 * k + c]);
+}
+
+// The merge's statistics of the current labels without a pixel pass: the
+// live statistics (kept exact by the sweeps, the split and the merge) moved
+// along the relabel map of kb_relabel_apply (a label L goes to target[L]
+// when that is set, in one hop; propagated frames only).
+__global__ void kb_merge_gather(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  const LabelStats& st = F.st;
+  const int n = st.count[k];
+  if (n == 0) return;
+  const int x = (F.prop_mode && F.target[k] >= 0) ? F.target[k] : k;
+  const LabelStats& h = F.sst;
+  atomicAdd(h.count + x, n);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    atomic_add_i64(h.S + 3 * x + c, st.S[3 * k + c]);
+    atomic_add_i64(h.Q + 3 * x + c, st.Q[3 * k + c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) atomic_add_i64(h.X + 5 * x + c, st.X[5 * k + c]);
 }
 
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
-  kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 2);
+  kb_merge_gather<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_marg<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
