@@ -1350,30 +1350,19 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
 // ---------------------------------------------------------------------------
 // merge (split_merge_orig.cu:79-199, split_merge_prop.cu:87-181)
 // ---------------------------------------------------------------------------
-__global__ void kb_border(Batch b, int cur) {
-  PIX_PROLOGUE
-  if (t >= F.npix) return;
-  const int W = F.W, H = F.H;
-  const int* seg = F.seg[cur];
-  int x = t % W, y = t / W;
-  uint8_t bb;
-  if (y < 1 || x < 1 || y >= H - 1 || x >= W - 1) {
-    bb = 1;
-  } else {
-    int C = seg[t];
-    bb = (C != seg[t - W]) || (C != seg[t + W]) || (C != seg[t + 1]) || (C != seg[t - 1]);
-  }
-  F.border[t] = bb;
-}
-
+// Merge candidates of border pixels (a pixel on the frame edge or with a
+// 4-neighbour of another label), the border test done inline.
 __global__ void kb_merge_cand(Batch b, int cur, int direction) {
   PIX_PROLOGUE
-  if (t >= F.npix || !F.border[t]) return;
+  if (t >= F.npix) return;
   const int W = F.W, H = F.H;
   const int* seg = F.seg[cur];
   const int prop_mode = F.prop_mode;
   int x = t % W, y = t / W;
   int C = seg[t], Wn = -1;
+  if (y >= 1 && x >= 1 && y < H - 1 && x < W - 1 && C == seg[t - W] && C == seg[t + W] &&
+      C == seg[t + 1] && C == seg[t - 1])
+    return;
   if (direction == 1) {
     bool ok = prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
     if (ok) Wn = seg[t + W];
@@ -1533,7 +1522,6 @@ __global__ void kb_merge_gather(Batch b) {
 
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
-  kb_border<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
   kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
   kb_merge_gather<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
