@@ -89,6 +89,7 @@ inline int stat_tiles(int H, int W) {
 
 // refinement-loop launches over a batch (k_batch.hip)
 void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s);
+void b_prepare_split(const Batch& b, int cur, hipStream_t s);
 void b_param_only(const Batch& b, hipStream_t s);
 void b_param_summary(const Batch& b, int cur, hipStream_t s);
 void b_store_sample(const Batch& b, hipStream_t s);

@@ -641,14 +641,16 @@ static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& sid
 }
 
 // The parameter update at the start of an iteration.  The sweeps keep the
-// label statistics current (a changed pixel moves its integer sums), so a
-// full recount is needed only at the first iteration (the prologue leaves
-// no statistics) and before a split (it also gathers the bounding boxes);
-// the updates after split and merge recount as well.
+// label statistics current (a changed pixel moves its integer sums) and
+// split and merge combine them exactly, so a full recount is needed only at
+// the first iteration (the prologue leaves no statistics); before a split a
+// lighter pass gathers the bounding boxes.
 static void update_at_start(const Batch& b, const stsp_config& cfg, int idx, int cur,
                             hipStream_t s) {
-  if (idx == 0 || is_split_iter(cfg, idx))
-    b_update_params(b, cur, is_split_iter(cfg, idx), s);
+  if (idx == 0)
+    b_update_params(b, cur, false, s);
+  else if (is_split_iter(cfg, idx))
+    b_prepare_split(b, cur, s);
   else
     b_param_only(b, s);
 }

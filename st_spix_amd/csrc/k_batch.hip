@@ -85,7 +85,9 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 // + sum dx^2, sum xy = n*x0*y0 + x0*sum dy + y0*sum dx + sum dx*dy.
 // SQ: also gather the colour squares (only the split and merge passes read
 // them; the live-label statistics never do).
-template <bool BBOX, bool SQ>
+// SUMS = false: bounding boxes (and the halves copy) only, for the pass
+// before a split, when the sums are already current
+template <bool BBOX, bool SQ, bool SUMS = true>
 __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int which) {
   const int f = frame_of(b.off.st, b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
@@ -119,11 +121,13 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       if (v >= 0 && v < nlab) {
         k[j] = v;
         pend |= 1u << j;
-        const float* p = F.img + 3 * t;
-        col[j][0] = p[0];
-        col[j][1] = p[1];
-        col[j][2] = p[2];
-        if (shifted) inv[j] = shifted[t] < 0;
+        if (SUMS) {
+          const float* p = F.img + 3 * t;
+          col[j][0] = p[0];
+          col[j][1] = p[1];
+          col[j][2] = p[2];
+          if (shifted) inv[j] = shifted[t] < 0;
+        }
       }
     }
   }
@@ -142,16 +146,18 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
         pend &= ~(1u << j);
         const int dx = lx + j;
         cnt += 1;
-        sdx += dx;
-        sdy += ly;
-        sxx += dx * dx;
-        sxy += dx * ly;
-        syy += ly * ly;
-        ninv += inv[j];
+        if (SUMS) {
+          sdx += dx;
+          sdy += ly;
+          sxx += dx * dx;
+          sxy += dx * ly;
+          syy += ly * ly;
+          ninv += inv[j];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-          S[c] += fx_color(col[j][c]);
-          if (SQ) Q[c] += fx_square(col[j][c]);
+          for (int c = 0; c < 3; c++) {
+            S[c] += fx_color(col[j][c]);
+            if (SQ) Q[c] += fx_square(col[j][c]);
+          }
         }
         if (BBOX) {
           mnx = min(mnx, tx0 + dx);
@@ -160,17 +166,19 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       }
     }
     int mny = cnt ? y : 0x7fffffff, mxy = cnt ? y : -1;
-    cnt = wave_sum_to63(cnt);
-    sdx = wave_sum_to63(sdx);
-    sdy = wave_sum_to63(sdy);
-    sxx = wave_sum_to63(sxx);
-    sxy = wave_sum_to63(sxy);
-    syy = wave_sum_to63(syy);
-    if (shifted) ninv = wave_sum_to63(ninv);
+    if (SUMS) {
+      cnt = wave_sum_to63(cnt);
+      sdx = wave_sum_to63(sdx);
+      sdy = wave_sum_to63(sdy);
+      sxx = wave_sum_to63(sxx);
+      sxy = wave_sum_to63(sxy);
+      syy = wave_sum_to63(syy);
+      if (shifted) ninv = wave_sum_to63(ninv);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      S[c] = wave_sum64_to63(S[c]);
-      if (SQ) Q[c] = wave_sum64_to63(Q[c]);
+      for (int c = 0; c < 3; c++) {
+        S[c] = wave_sum64_to63(S[c]);
+        if (SQ) Q[c] = wave_sum64_to63(Q[c]);
+      }
     }
     if (BBOX) {
       mnx = wave_min_to63(mnx);
@@ -178,7 +186,13 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       mny = wave_min_to63(mny);
       mxy = wave_max_to63(mxy);
     }
-    if (lane == 63) {
+    if (lane == 63 && !SUMS && BBOX) {
+      atomicMin(st.bbox + 4 * u, mnx);
+      atomicMax(st.bbox + 4 * u + 1, mxx);
+      atomicMin(st.bbox + 4 * u + 2, mny);
+      atomicMax(st.bbox + 4 * u + 3, mxy);
+    }
+    if (lane == 63 && SUMS) {
       const int64_t n = cnt, X0 = tx0, Y0 = ty0;
       atomicAdd(st.count + u, cnt);
       if (ninv) atomicAdd(st.ninvalid + u, ninv);
@@ -335,6 +349,25 @@ void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
     kb_stats_accum<true, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   else
     kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+  kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
+}
+
+__global__ void kb_bbox_clear(Batch b) {
+  LAB_PROLOGUE
+  if (k >= F.nbuf) return;
+  int* bb = F.st.bbox + 4 * k;
+  bb[0] = 0x7fffffff;
+  bb[1] = -1;
+  bb[2] = 0x7fffffff;
+  bb[3] = -1;
+}
+
+// Before a split: the label sums are current (the sweeps move them, split and
+// merge combine them), so the pass gathers only the bounding boxes and
+// starts the halves map; then the parameters are updated.
+void b_prepare_split(const Batch& b, int cur, hipStream_t s) {
+  kb_bbox_clear<<<b.lab_blocks, kBlock, 0, s>>>(b);
+  kb_stats_accum<true, false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
