@@ -151,12 +151,17 @@ def run_served(args, streams, torch, core):
     # the serving stream (no event between launches, so no dispatch bubbles)
     launches = SWEEP_TIMING_LAUNCHES
     ms = srv.runner.time_sweeps(launches, stream=srv.cuda_stream)
+    search = {}
+    for st in srv.streams.values():  # split-proposal searches per path, per frame size
+        key = f"{st.H}x{st.W}"
+        c = st.ctx.split_search_counts()
+        search[key] = {k: search.get(key, {}).get(k, 0) + v for k, v in c.items()}
     px = sum(s["H"] * s["W"] for s in streams)
     per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
     srv.close()
     return {"elapsed": elapsed, "lat": lat, "sweep_ms": ms,
             "sweep_bytes": launches * BYTES_PER_PX_SWEEP * per_launch * px,
-            "sweep_launches": launches}
+            "sweep_launches": launches, "split_search": search}
 
 
 def cpu_baseline(args):
@@ -236,6 +241,7 @@ def main():
     log(f"{len(streams)} synthetic streams resident in HBM")
     served = run_served(args, streams, torch, core)
     log(f"served: {served['elapsed']:.3f} s for {args.steps} steps")
+    log(f"split searches per path: {served['split_search']}")
     plain = None if args.no_plain else run_plain(args, streams, torch, core)
     if plain is not None:
         log(f"plain loop: {plain['elapsed']:.3f} s")

@@ -93,6 +93,23 @@ int stsp_ctx_set_config(stsp_ctx* ctx, const stsp_config* cfg);
  * differs.  Returns STSP_ERR_ARG outside 0..5. */
 int stsp_ctx_set_tuning(stsp_ctx* ctx, int sweep_shape);
 
+/* Split-proposal search limits (no counterpart in the reference, whose
+ * search is a level-synchronous flood fill over the whole frame,
+ * split_merge_prop.cu:277-323).  Bounding boxes up to wave_box x wave_box
+ * pixels (1..128) are searched by one wavefront; larger ones by a workgroup
+ * in a window of at most win_rows x win_cols pixels (1..448, 1..384) around
+ * the first seed, then over the whole box if the seed's connected component
+ * leaves the window.  Results are identical for every setting; only speed
+ * differs.  Defaults: 128, 448, 384. */
+int stsp_ctx_set_split_search(stsp_ctx* ctx, int wave_box, int win_rows, int win_cols);
+
+/* Cumulative number of split-proposal searches by path since the context was
+ * created: [0] one wavefront, [1] workgroup over the whole box, [2] workgroup
+ * window that held the seed's component, [3] whole box after the component
+ * left the window.  Waits for the device.  Diagnostic; no reference
+ * counterpart. */
+int stsp_ctx_split_search_counts(stsp_ctx* ctx, long long counts[4]);
+
 /* Capacity an output parameter set needs: BASS 30*K0 (bass.cu:245),
  * propagated frame 10*nspix_prev (prop.cu:702). */
 int stsp_bass_capacity(int sp_size, int height, int width);

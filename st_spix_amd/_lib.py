@@ -62,7 +62,8 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
            "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
            "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_prop_batch", "stsp_bass_batch",
-           "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb"]
+           "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb", "stsp_ctx_set_split_search",
+           "stsp_ctx_split_search_counts"]
 
 
 def lib() -> ctypes.CDLL:
@@ -90,6 +91,8 @@ def lib() -> ctypes.CDLL:
         L.stsp_ctx_kernel_time.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_longlong)]
         L.stsp_ctx_set_tuning.argtypes = [vp, i]
+        L.stsp_ctx_set_split_search.argtypes = [vp, i, i, i]
+        L.stsp_ctx_split_search_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.stsp_batch_create.argtypes = [i, ctypes.POINTER(vp)]
         L.stsp_batch_destroy.argtypes = [vp]
         L.stsp_batch_destroy.restype = None

@@ -56,6 +56,21 @@ class Context:
         _lib.check(_lib.lib().stsp_ctx_set_tuning(self.handle, int(sweep_shape)),
                    "stsp_ctx_set_tuning")
 
+    def set_split_search(self, wave_box: int = 128, win_rows: int = 448,
+                         win_cols: int = 384) -> None:
+        """Limits of the split proposal's search paths (stsp_ctx_set_split_search);
+        results are the same for every setting."""
+        _lib.check(_lib.lib().stsp_ctx_set_split_search(self.handle, int(wave_box),
+                                                        int(win_rows), int(win_cols)),
+                   "stsp_ctx_set_split_search")
+
+    def split_search_counts(self) -> Dict[str, int]:
+        """Cumulative split-proposal searches per path (waits for the device)."""
+        out = (ctypes.c_longlong * 4)()
+        _lib.check(_lib.lib().stsp_ctx_split_search_counts(self.handle, out),
+                   "stsp_ctx_split_search_counts")
+        return dict(zip(("wave", "band", "window", "whole"), (int(v) for v in out)))
+
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(_lib.lib().stsp_ctx_enable_timing(self.handle, int(on)),
                    "stsp_ctx_enable_timing")
@@ -170,19 +185,31 @@ def bass_frame(img: torch.Tensor, cfg: _lib.Config,
     return spix, trim(out, n.value)
 
 
+def check_prop_inputs(ctx: Optional["Context"], img: torch.Tensor, flow: torch.Tensor,
+                      spix_prev: torch.Tensor) -> None:
+    """Device, layout, dtype and shape checks of one propagated frame's inputs
+    (the kernels index them with the context's frame size)."""
+    for t, name in ((img, "img"), (flow, "flow"), (spix_prev, "spix_prev")):
+        _check_input(t, name)
+    if img.dtype != torch.float32 or img.dim() != 3 or img.shape[2] != 3:
+        raise RuntimeError("img must be float32 (H, W, 3)")
+    H, W = img.shape[:2]
+    if ctx is not None and (H, W) != (ctx.H, ctx.W):
+        raise RuntimeError(f"img is {H}x{W} but its context is {ctx.H}x{ctx.W}")
+    if flow.shape != (H, W, 2) or flow.dtype != torch.float32:
+        raise RuntimeError("flow must be float32 (H, W, 2)")
+    if spix_prev.shape != (H, W) or spix_prev.dtype != torch.int32:
+        raise RuntimeError("spix_prev must be int32 (H, W)")
+
+
 def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
                params_prev: Params, cfg: _lib.Config,
                ctx: Optional[Context] = None) -> Tuple[torch.Tensor, Params, int]:
     """One propagated frame.  ``params_prev['prior_counts']`` and
     ``params_prev['invalid_counts']`` are updated in place, as the reference's
     shift_and_fill does.  Returns (spix, params, adaptive_niters)."""
-    for t, name in ((img, "img"), (flow, "flow"), (spix_prev, "spix_prev")):
-        _check_input(t, name)
+    check_prop_inputs(ctx, img, flow, spix_prev)
     H, W = img.shape[:2]
-    if flow.shape != (H, W, 2) or flow.dtype != torch.float32:
-        raise RuntimeError("flow must be float32 (H, W, 2)")
-    if spix_prev.shape != (H, W) or spix_prev.dtype != torch.int32:
-        raise RuntimeError("spix_prev must be int32 (H, W)")
     nprev = int(params_prev["ids"].shape[0])
     if ctx is None:
         ctx = get_context(H, W, cfg, img.device)
@@ -311,8 +338,7 @@ class BatchRunner:
         niters = (ctypes.c_int * n)()
         keep = []
         for i, (ctx, img, flow, spix_prev, params_prev) in enumerate(items):
-            for t, name in ((img, "img"), (flow, "flow"), (spix_prev, "spix_prev")):
-                _check_input(t, name)
+            check_prop_inputs(ctx, img, flow, spix_prev)
             np_ = int(params_prev["ids"].shape[0])
             cap = _lib.lib().stsp_prop_capacity(np_)
             out = alloc_params(cap, img.device, zero=False)

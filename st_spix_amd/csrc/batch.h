@@ -30,6 +30,10 @@ struct FrameDesc {
   int* max_spix;
   int* total;
   int* counter;           // per-frame list counters (dead list length)
+  // split proposal: boxes up to wave_box x wave_box are searched by one wave;
+  // larger boxes by a workgroup in a window of at most win_h x win_w pixels
+  int wave_box, win_h, win_w;
+  unsigned long long* search_count;  // cumulative searches per path (kSearchPaths)
   // relabel
   const float* prev_prior_mu_app;
   int nspix_prev;
@@ -44,6 +48,14 @@ struct FrameDesc {
   int sp_size, prop_mode;
   NigConsts nc;
 };
+
+// Split-proposal search paths counted per frame: one wave; the workgroup
+// bands over the whole box; the bands in a window that held seed 1's
+// component; the whole box after the component left the window.
+enum { kSearchWave = 0, kSearchBand, kSearchWindow, kSearchWhole, kSearchPaths };
+constexpr int kWaveBoxMax = 128;
+constexpr int kSplitWinRows = 896, kSplitWinCols = 512;  // the band search's register rows
+constexpr int kBlockRowsMax = 448, kBlockColsMax = 384;  // boxes of the 256-thread band kernel
 
 // Launch shapes of the fused sweep: tile width, height, sweeps per launch,
 // threads.  A remainder of fewer sweeps than a shape fuses runs shape 0.
@@ -64,7 +76,8 @@ struct BatchOff {
   int* st;       // statistics tiles (kStatTile x kStatTile pixels)
 };
 
-constexpr int kOffArrays = 4 + kSweepShapes;  // pix, lab, sm, tile[], st
+constexpr int kOffArrays = 4 + kSweepShapes;
+constexpr int kWorkLists = 3, kWorkCounters = 2 * kWorkLists;  // pix, lab, sm, tile[], st
 constexpr int kStatTile = 32;
 
 struct Batch {
@@ -72,8 +85,8 @@ struct Batch {
   BatchOff off;
   int nf;
   int pix_blocks, lab_blocks, sm_blocks, tile_blocks[kSweepShapes], st_blocks;
-  int* work;      // two work lists (frame << 24 | label) of work_cap entries each
-  int* work_n;    // [0] length, [1] cursor of list 0; [2], [3] of list 1
+  int* work;      // kWorkLists work lists (frame << 24 | label) of work_cap entries each
+  int* work_n;    // [2i] length, [2i + 1] cursor of list i
   int work_cap;
 };
 
@@ -98,8 +111,8 @@ int b_sweeps(const Batch& b, int cur, int niters_seg, int shape, const SimpleLut
 // A second stream with fork/join events: launches issued on it overlap the
 // main stream's between the fork and the join (also inside a graph capture).
 struct SideStream {
-  hipStream_t st;
-  hipEvent_t fork, join;
+  hipStream_t st, st2;
+  hipEvent_t fork, join, join2;
 };
 void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side);
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s);

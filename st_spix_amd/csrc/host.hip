@@ -109,12 +109,26 @@ struct Lanes {
   int n = 0;
 };
 
-// A batch table on the device plus the cached refinement graph.
+// A refinement graph captured for one batch composition.
+struct CachedGraph {
+  std::vector<long long> key;
+  hipGraphExec_t exec = nullptr;
+  int cur = 0;        // live label buffer after the graph's launches
+  long long used = 0; // last use (eviction order)
+};
+constexpr int kGraphCache = 8;
+
+// Batch tables on the device plus the cached refinement graphs.  Frames of
+// one call that do not share a schedule are refined in groups; group g uses
+// its own slice of the tables (frames [base, base + n) of h_fr/d_fr and
+// offset slice g of h_off/d_off), so a group's table copy, which runs in
+// stream order after the previous group's work, never reads host memory that
+// a later group has rewritten.
 struct stsp_batch {
-  SideStream side = {nullptr, nullptr, nullptr};
+  SideStream side = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int maxf = 0;
   FrameDesc* d_fr = nullptr;
-  int* d_off = nullptr;  // kOffArrays arrays of maxf+1
+  int* d_off = nullptr;  // maxf slices of kOffArrays arrays of maxf+1
   FrameDesc* h_fr = nullptr;
   int* h_off = nullptr;
   int* d_work = nullptr;
@@ -122,9 +136,8 @@ struct stsp_batch {
   int work_cap = 0;
   Batch b;
   bool use_graph = false;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<long long> gkey;
-  int gcur = 0;
+  std::vector<CachedGraph> graphs;
+  long long graph_clock = 0;
   Lanes lanes;
   // the last refinement's launch settings (for stsp_batch_time_sweeps)
   SimpleLut last_lut = {};
@@ -170,6 +183,8 @@ struct stsp_ctx {
   unsigned long long* best;
   // scalars
   int *max_spix, *total, *num_new, *counter;
+  unsigned long long* search_count;  // kSearchPaths cumulative counters
+  int wave_box = kWaveBoxMax, win_h = 0, win_w = 0;  // split search limits (0: widest)
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
@@ -207,16 +222,17 @@ static int alloc_pixels(stsp_ctx* c) {
   c->total = a.take<int>(1);
   c->num_new = a.take<int>(1);
   c->counter = a.take<int>(4);
+  c->search_count = a.take<unsigned long long>(kSearchPaths);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
 }
 
+// Drops every cached graph.  Callers make sure none is in flight (after a
+// device or stream synchronisation, or before any was launched).
 static void invalidate_graph(stsp_batch* bh) {
-  if (bh->gexec) {
-    (void)hipGraphExecDestroy(bh->gexec);
-    bh->gexec = nullptr;
-  }
-  bh->gkey.clear();
+  for (CachedGraph& g : bh->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  bh->graphs.clear();
 }
 
 // (re)allocate the per-label workspace for `cap` slots
@@ -330,10 +346,11 @@ static int ensure_prev(stsp_ctx* c, int n) {
 static int batch_alloc(stsp_batch* bh, int maxf) {
   bh->maxf = maxf;
   HIP_TRY(hipMalloc((void**)&bh->d_fr, sizeof(FrameDesc) * maxf));
-  HIP_TRY(hipMalloc((void**)&bh->d_off, sizeof(int) * kOffArrays * (maxf + 1)));
-  HIP_TRY(hipMalloc((void**)&bh->d_work_n, sizeof(int) * 4));
+  const size_t off_bytes = sizeof(int) * kOffArrays * (maxf + 1) * maxf;
+  HIP_TRY(hipMalloc((void**)&bh->d_off, off_bytes));
+  HIP_TRY(hipMalloc((void**)&bh->d_work_n, sizeof(int) * kWorkCounters));
   HIP_TRY(hipHostMalloc((void**)&bh->h_fr, sizeof(FrameDesc) * maxf, hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void**)&bh->h_off, sizeof(int) * kOffArrays * (maxf + 1), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_off, off_bytes, hipHostMallocDefault));
   HIP_TRY(hipMalloc((void**)&bh->d_pro, sizeof(ProFrame) * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
   HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
@@ -407,10 +424,13 @@ static void batch_free(stsp_batch* bh) {
   lanes_free(&bh->lanes);
   if (bh->side.st) {
     (void)hipStreamSynchronize(bh->side.st);
+    (void)hipStreamSynchronize(bh->side.st2);
     (void)hipStreamDestroy(bh->side.st);
+    (void)hipStreamDestroy(bh->side.st2);
     (void)hipEventDestroy(bh->side.fork);
     (void)hipEventDestroy(bh->side.join);
-    bh->side = {nullptr, nullptr, nullptr};
+    (void)hipEventDestroy(bh->side.join2);
+    bh->side = {nullptr, nullptr, nullptr, nullptr, nullptr};
   }
   invalidate_graph(bh);
   if (bh->d_fr) (void)hipFree(bh->d_fr);
@@ -522,6 +542,10 @@ static FrameDesc make_desc(stsp_ctx* c) {
   d.max_spix = c->max_spix;
   d.total = c->total;
   d.counter = c->counter;
+  d.wave_box = c->wave_box;
+  d.win_h = c->win_h > 0 ? c->win_h : kSplitWinRows;
+  d.win_w = c->win_w > 0 ? c->win_w : kSplitWinCols;
+  d.search_count = c->search_count;
   d.prev_prior_mu_app = c->prev_prior_mu_app;
   d.nspix_prev = c->nspix_prev;
   d.alive = c->alive;
@@ -542,17 +566,40 @@ static FrameDesc make_desc(stsp_ctx* c) {
   return d;
 }
 
-// Fills the device table for frames ctxs[0..n); grid sizes cover each
-// frame's capacity so that a captured graph stays valid across frames.
-static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
-  if (n > bh->maxf || n > 64) return STSP_ERR_ARG;
+// Work lists (split candidates, relabel items) for `work` labels; grows only
+// with the stream idle, and drops the graphs that captured the old lists.
+static int ensure_work(stsp_batch* bh, int work, hipStream_t s) {
+  if (work <= bh->work_cap) return STSP_OK;
+  if (bh->d_work) {
+    HIP_TRY(hipStreamSynchronize(s));
+    if (bh->side.st) HIP_TRY(hipStreamSynchronize(bh->side.st));
+    if (bh->side.st2) HIP_TRY(hipStreamSynchronize(bh->side.st2));
+    HIP_TRY(hipFree(bh->d_work));
+    bh->d_work = nullptr;
+  }
+  invalidate_graph(bh);
+  HIP_TRY(hipMalloc((void**)&bh->d_work, kWorkLists * sizeof(int) * (size_t)work));
+  bh->work_cap = work;
+  return STSP_OK;
+}
+
+// Fills the device table of group `slot` for frames ctxs[0..n), stored at
+// frames [base, base + n) of the frame table; grid sizes cover each frame's
+// capacity so that a captured graph stays valid across frames.
+static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s, int base = 0,
+                       int slot = 0) {
+  if (n < 1 || base < 0 || slot < 0 || base + n > bh->maxf || slot >= bh->maxf || n > 64)
+    return STSP_ERR_ARG;
   if (!bh->side.st) {
     HIP_TRY(hipStreamCreateWithFlags(&bh->side.st, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&bh->side.st2, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&bh->side.fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&bh->side.join, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&bh->side.join2, hipEventDisableTiming));
   }
   const int M = bh->maxf + 1;
-  int* off = bh->h_off;
+  int* off = bh->h_off + (size_t)slot * kOffArrays * M;
+  FrameDesc* hfr = bh->h_fr + base;
   int* o_pix = off;
   int* o_lab = off + M;
   int* o_sm = off + 2 * M;
@@ -564,7 +611,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   for (int sh = 0; sh < kSweepShapes; sh++) o_t[sh][0] = 0;
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = ctxs[i];
-    bh->h_fr[i] = make_desc(c);
+    hfr[i] = make_desc(c);
     o_pix[i + 1] = o_pix[i] + div_up(c->npix, kBlock);
     o_lab[i + 1] = o_lab[i] + div_up(c->cap, kBlock);
     o_sm[i + 1] = o_sm[i] + div_up(2 * c->cap + 2, kBlock);
@@ -573,20 +620,13 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
     o_st[i + 1] = o_st[i] + stat_tiles(c->H, c->W);
     work += c->cap;
   }
-  if (work > bh->work_cap) {
-    if (bh->d_work) {
-      HIP_TRY(hipStreamSynchronize(s));
-      HIP_TRY(hipFree(bh->d_work));
-    }
-    HIP_TRY(hipMalloc((void**)&bh->d_work, 2 * sizeof(int) * (size_t)work));  // two lists
-    bh->work_cap = work;
-    invalidate_graph(bh);
-  }
-  HIP_TRY(hipMemcpyAsync(bh->d_fr, bh->h_fr, sizeof(FrameDesc) * n, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(bh->d_off, bh->h_off, sizeof(int) * kOffArrays * M, hipMemcpyHostToDevice, s));
+  int rc = ensure_work(bh, work, s);
+  if (rc) return rc;
+  int* d = bh->d_off + (size_t)slot * kOffArrays * M;
+  HIP_TRY(hipMemcpyAsync(bh->d_fr + base, hfr, sizeof(FrameDesc) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d, off, sizeof(int) * kOffArrays * M, hipMemcpyHostToDevice, s));
   Batch& b = bh->b;
-  b.fr = bh->d_fr;
-  int* d = bh->d_off;
+  b.fr = bh->d_fr + base;
   b.off.pix = d;
   b.off.lab = d + M;
   b.off.sm = d + 2 * M;
@@ -647,8 +687,8 @@ static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& sid
 // lighter pass gathers the bounding boxes.
 static void update_at_start(const Batch& b, const stsp_config& cfg, int idx, int cur,
                             hipStream_t s) {
-  if (idx == 0)
-    b_update_params(b, cur, false, s);
+  if (idx == 0)  // bounding boxes and the halves map too if it is a split iteration
+    b_update_params(b, cur, is_split_iter(cfg, idx), s);
   else if (is_split_iter(cfg, idx))
     b_prepare_split(b, cur, s);
   else
@@ -741,8 +781,10 @@ static int refine_controlled(stsp_batch* bh, stsp_ctx* c, hipStream_t s) {
   return STSP_OK;
 }
 
-// Runs the refinement of frames ctxs[0..n) (prologues done) as one batch.
-static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
+// Runs the refinement of frames ctxs[0..n) (prologues done) as one batch,
+// with the tables of group `slot` at frame `base` (see stsp_batch).
+static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s, int base = 0,
+                  int slot = 0) {
   stsp_ctx* lead = ctxs[0];
   if (lead->cfg.target_nspix > 0) return n == 1 ? refine_controlled(bh, lead, s) : STSP_ERR_ARG;
   const int niters = lead->niters, prop_mode = lead->prop_mode, shape = lead->sweep_shape;
@@ -753,7 +795,7 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
         a.niters_seg != l.niters_seg || a.sm_period != l.sm_period || a.sm_start != l.sm_start)
       return STSP_ERR_ARG;
   }
-  int rc = batch_build(bh, ctxs, n, s);
+  int rc = batch_build(bh, ctxs, n, s, base, slot);
   if (rc) return rc;
   bh->last_lut = lead->lut;
   bh->last_shape = shape;
@@ -763,13 +805,27 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
   // eagerly (same kernels, same results).
   if (bh->use_graph && !timing && s != nullptr) {
     std::vector<long long> key = {n, niters, prop_mode, shape, lead->cfg.niters_seg,
-                                  lead->cfg.sm_period, lead->cfg.sm_start, bh->work_cap};
+                                  lead->cfg.sm_period, lead->cfg.sm_start, bh->work_cap,
+                                  base, slot};
     for (int i = 0; i < n; i++) {
-      key.push_back((long long)(size_t)ctxs[i]);
+      key.push_back((long long)(size_t)ctxs[i]);  // grid sizes follow from cap, H and W
       key.push_back(ctxs[i]->cap);
+      key.push_back(ctxs[i]->H);
+      key.push_back(ctxs[i]->W);
     }
-    if (!bh->gexec || key != bh->gkey) {
-      invalidate_graph(bh);
+    CachedGraph* hit = nullptr;
+    for (CachedGraph& g : bh->graphs)
+      if (g.key == key) hit = &g;
+    if (!hit) {
+      if ((int)bh->graphs.size() >= kGraphCache) {
+        // evict the least recently used graph once the stream has drained
+        size_t lru = 0;
+        for (size_t i = 1; i < bh->graphs.size(); i++)
+          if (bh->graphs[i].used < bh->graphs[lru].used) lru = i;
+        HIP_TRY(hipStreamSynchronize(s));
+        (void)hipGraphExecDestroy(bh->graphs[lru].exec);
+        bh->graphs.erase(bh->graphs.begin() + lru);
+      }
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int c0 = refine_launches(lead, bh->b, bh->side, niters, prop_mode, shape, s);
       hipGraph_t g = nullptr;
@@ -779,17 +835,18 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s) {
         if (g) (void)hipGraphDestroy(g);
         return STSP_ERR_HIP;
       }
-      const hipError_t ie = hipGraphInstantiate(&bh->gexec, g, nullptr, nullptr, 0);
+      CachedGraph cg;
+      const hipError_t ie = hipGraphInstantiate(&cg.exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
-      if (ie != hipSuccess) {
-        bh->gexec = nullptr;
-        return STSP_ERR_HIP;
-      }
-      bh->gkey = key;
-      bh->gcur = c0;
+      if (ie != hipSuccess) return STSP_ERR_HIP;
+      cg.key = key;
+      cg.cur = c0;
+      bh->graphs.push_back(cg);
+      hit = &bh->graphs.back();
     }
-    HIP_TRY(hipGraphLaunch(bh->gexec, s));
-    cur = bh->gcur;
+    hit->used = ++bh->graph_clock;
+    HIP_TRY(hipGraphLaunch(hit->exec, s));
+    cur = hit->cur;
   } else {
     cur = refine_launches(lead, bh->b, bh->side, niters, prop_mode, shape, s);
   }
@@ -1131,6 +1188,8 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
   c->lut = make_lut();
   int rc = alloc_pixels(c);
   if (!rc) rc = batch_alloc(&c->self, 1);
+  if (!rc && hipMemset(c->search_count, 0, sizeof(unsigned long long) * kSearchPaths) != hipSuccess)
+    rc = STSP_ERR_HIP;
   if (rc) {
     stsp_ctx_destroy(c);
     return rc;
@@ -1161,6 +1220,25 @@ int stsp_ctx_set_config(stsp_ctx* c, const stsp_config* cfg) {
 int stsp_ctx_set_tuning(stsp_ctx* c, int sweep_shape) {
   if (!c || sweep_shape < 0 || sweep_shape >= kSweepShapes) return STSP_ERR_ARG;
   c->sweep_shape = sweep_shape;
+  return STSP_OK;
+}
+
+int stsp_ctx_set_split_search(stsp_ctx* c, int wave_box, int win_rows, int win_cols) {
+  if (!c || wave_box < 1 || wave_box > kWaveBoxMax || win_rows < 1 || win_rows > kSplitWinRows ||
+      win_cols < 1 || win_cols > kSplitWinCols)
+    return STSP_ERR_ARG;
+  c->wave_box = wave_box;
+  c->win_h = win_rows;
+  c->win_w = win_cols;
+  return STSP_OK;
+}
+
+int stsp_ctx_split_search_counts(stsp_ctx* c, long long* counts) {
+  if (!c || !counts) return STSP_ERR_ARG;
+  unsigned long long h[kSearchPaths];
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(h, c->search_count, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < kSearchPaths; i++) counts[i] = (long long)h[i];
   return STSP_OK;
 }
 
@@ -1260,6 +1338,12 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
                              stsp_params* outs, int* nspix_out, hipStream_t s) {
   std::vector<int> done(n, 0);
   std::vector<stsp_ctx*> group;
+  // the work lists of the largest group are allocated before any group runs
+  int work = 0;
+  for (int i = 0; i < n; i++) work += ctxs[i]->cap;
+  int rc0 = ensure_work(bh, work, s);
+  if (rc0) return rc0;
+  int base = 0, slot = 0;
   for (int i = 0; i < n; i++) {
     if (done[i]) continue;
     group.clear();
@@ -1273,8 +1357,10 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
       group.push_back(ctxs[j]);
       done[j] = 1;
     }
-    int rc = refine(bh, group.data(), (int)group.size(), s);
+    int rc = refine(bh, group.data(), (int)group.size(), s, base, slot);
     if (rc) return rc;
+    base += (int)group.size();
+    slot++;
   }
   Lanes* L = &bh->lanes;
   int rc = lanes_fork(L, s, n);
