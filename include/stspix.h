@@ -15,6 +15,7 @@
 #define STSPIX_H
 
 #include <hip/hip_runtime_api.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -27,7 +28,8 @@ typedef enum {
   STSP_ERR_CAPACITY = -2, /* output capacity smaller than the result */
   STSP_ERR_FILL = -3,     /* hole filling stopped shrinking (reference exits) */
   STSP_ERR_HIP = -4,      /* a HIP runtime call failed */
-  STSP_ERR_STATE = -5     /* stream step without a previous frame */
+  STSP_ERR_STATE = -5,    /* stream step without a previous frame */
+  STSP_ERR_TOO_LARGE = -6 /* frame exceeds stsp_bass_noalloc's static scratch buffer */
 } stsp_status;
 
 /* Algorithm settings; defaults are the reference's (pyapi.cu:63-73,
@@ -152,6 +154,22 @@ int stsp_prop_batch(stsp_batch* batch, stsp_ctx** ctxs, int n, const float** img
  * indepent_bass, prop/__init__.py:38-85). */
 int stsp_bass_batch(stsp_batch* batch, stsp_ctx** ctxs, int n, const float** imgs,
                     int** spix_out, stsp_params* outs, int* nspix_out, hipStream_t stream);
+
+/* Single-image BASS that allocates nothing: the same computation and result
+ * as stsp_bass (run_bass, bass.cu:221-346) for a context created with cfg at
+ * height x width, with the whole workspace in a static device buffer of
+ * 32 MiB (one copy per device, held until the process ends) instead of a
+ * context.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Returns STSP_ERR_TOO_LARGE when the frame's workspace
+ * (stsp_bass_noalloc_scratch_bytes) exceeds the buffer, STSP_ERR_ARG for
+ * target_nspix > 0.  All calls on a device share the buffer: calls whose
+ * device work overlaps race, so order them (one stream, or synchronise). */
+int stsp_bass_noalloc(const stsp_config* cfg, int height, int width, const float* img,
+                      int* spix_out, stsp_params* out, int* nspix_out, hipStream_t stream);
+/* Scratch bytes stsp_bass_noalloc needs for this configuration and size. */
+int stsp_bass_noalloc_scratch_bytes(const stsp_config* cfg, int height, int width,
+                                    size_t* bytes);
 
 /* Border overlay of get_marked_video (pyapi.cu:221-303,
  * find_border_pixels_end seg_utils.cu:218-257): marked = clamp(vid,0,1)

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libstspix.so")
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "output capacity too small",
           -3: "hole filling stopped shrinking", -4: "HIP runtime error",
-          -5: "invalid stream state"}
+          -5: "invalid stream state", -6: "frame exceeds the static scratch buffer"}
 
 
 class StspError(RuntimeError):
@@ -63,7 +63,7 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
            "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_prop_batch", "stsp_bass_batch",
            "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb", "stsp_ctx_set_split_search",
-           "stsp_ctx_split_search_counts"]
+           "stsp_ctx_split_search_counts", "stsp_bass_noalloc", "stsp_bass_noalloc_scratch_bytes"]
 
 
 def lib() -> ctypes.CDLL:
@@ -93,6 +93,10 @@ def lib() -> ctypes.CDLL:
         L.stsp_ctx_set_tuning.argtypes = [vp, i]
         L.stsp_ctx_set_split_search.argtypes = [vp, i, i, i]
         L.stsp_ctx_split_search_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.stsp_bass_noalloc.argtypes = [ctypes.POINTER(Config), i, i, vp, vp,
+                                        ctypes.POINTER(ParamsStruct), pi, vp]
+        L.stsp_bass_noalloc_scratch_bytes.argtypes = [ctypes.POINTER(Config), i, i,
+                                                      ctypes.POINTER(ctypes.c_size_t)]
         L.stsp_batch_create.argtypes = [i, ctypes.POINTER(vp)]
         L.stsp_batch_destroy.argtypes = [vp]
         L.stsp_batch_destroy.restype = None

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -136,6 +137,7 @@ struct stsp_batch {
   int work_cap = 0;
   Batch b;
   bool use_graph = false;
+  bool noalloc = false;  // tables carved from the static scratch buffer, no side streams
   std::vector<CachedGraph> graphs;
   long long graph_clock = 0;
   Lanes lanes;
@@ -185,6 +187,9 @@ struct stsp_ctx {
   int *max_spix, *total, *num_new, *counter;
   unsigned long long* search_count;  // kSearchPaths cumulative counters
   int wave_box = kWaveBoxMax, win_h = 0, win_w = 0;  // split search limits (0: widest)
+  // workspace carved from the static scratch buffer: nothing is allocated,
+  // the side streams are not used and the capacity is fixed
+  bool noalloc = false;
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
@@ -202,12 +207,14 @@ struct stsp_ctx {
 // ---------------------------------------------------------------------------
 // allocation
 // ---------------------------------------------------------------------------
-static int alloc_pixels(stsp_ctx* c) {
+static size_t pixel_bytes(size_t n) { return 16 * 256 + n * (4 * 8 + 1 + 8) + 4096; }
+
+// the per-pixel buffers and device scalars of a context, from `base`
+static void carve_pixels(stsp_ctx* c, char* base) {
   size_t n = c->npix;
   Arena a;
-  a.size = 16 * 256 + n * (4 * 8 + 1 + 8) + 4096;
-  HIP_TRY(hipMalloc(&c->pix_mem, a.size));
-  a.base = (char*)c->pix_mem;
+  a.size = pixel_bytes(n);
+  a.base = base;
   c->seg = a.take<int>(n);
   c->seg2 = a.take<int>(n);
   c->halves = a.take<int>(n);
@@ -223,6 +230,11 @@ static int alloc_pixels(stsp_ctx* c) {
   c->num_new = a.take<int>(1);
   c->counter = a.take<int>(4);
   c->search_count = a.take<unsigned long long>(kSearchPaths);
+}
+
+static int alloc_pixels(stsp_ctx* c) {
+  HIP_TRY(hipMalloc(&c->pix_mem, pixel_bytes(c->npix)));
+  carve_pixels(c, (char*)c->pix_mem);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
 }
@@ -235,24 +247,19 @@ static void invalidate_graph(stsp_batch* bh) {
   bh->graphs.clear();
 }
 
-// (re)allocate the per-label workspace for `cap` slots
-static int ensure_labels(stsp_ctx* c, int cap) {
-  if (cap <= c->cap) return STSP_OK;
-  if (c->lab_mem) {
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipFree(c->lab_mem));
-    c->lab_mem = nullptr;
-  }
-  cap = cap + cap / 4;  // headroom for growth across frames
+static size_t label_bytes(int cap) {
   size_t n = cap, nsm = 2 * n + 2;
-  size_t bytes = n * (3 * 4 * 2 + 8 * (2 + 3 + 2 + 3 + 3 + 3 + 1) + 4 * 6) +
-                 n * (4 + 8 * 11 + 4 + 16) + nsm * (4 + 8 * 11 + 4) +
-                 nsm * (4 * 4 + 8 * 3 * 2 + 8 * 2 + 4 * 8) + nsm * 8 + n * sizeof(SweepRec) +
-                 n * 4 * 8 + compact_tmp_bytes(cap) + 100 * 256;
-  HIP_TRY(hipMalloc(&c->lab_mem, bytes));
+  return n * (3 * 4 * 2 + 8 * (2 + 3 + 2 + 3 + 3 + 3 + 1) + 4 * 6) + n * (4 + 8 * 11 + 4 + 16) +
+         nsm * (4 + 8 * 11 + 4) + nsm * (4 * 4 + 8 * 3 * 2 + 8 * 2 + 4 * 8) + nsm * 8 +
+         n * sizeof(SweepRec) + n * 4 * 8 + compact_tmp_bytes(cap) + 100 * 256;
+}
+
+// the per-label workspace of `cap` slots, from `base`
+static int carve_labels(stsp_ctx* c, char* base, int cap) {
+  size_t n = cap, nsm = 2 * n + 2;
   Arena a;
-  a.base = (char*)c->lab_mem;
-  a.size = bytes;
+  a.base = base;
+  a.size = label_bytes(cap);
   Params& P = c->P;
   P.mu_app = a.take<float>(3 * n);
   P.prior_mu_app = a.take<float>(3 * n);
@@ -315,8 +322,23 @@ static int ensure_labels(stsp_ctx* c, int cap) {
     return STSP_ERR_HIP;
   }
   c->cap = cap;
-  invalidate_graph(&c->self);
   return STSP_OK;
+}
+
+// (re)allocate the per-label workspace for `cap` slots
+static int ensure_labels(stsp_ctx* c, int cap) {
+  if (cap <= c->cap) return STSP_OK;
+  if (c->noalloc) return STSP_ERR_TOO_LARGE;  // sized once, from the static scratch buffer
+  if (c->lab_mem) {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(c->lab_mem));
+    c->lab_mem = nullptr;
+  }
+  cap = cap + cap / 4;  // headroom for growth across frames
+  HIP_TRY(hipMalloc(&c->lab_mem, label_bytes(cap)));
+  const int rc = carve_labels(c, (char*)c->lab_mem, cap);
+  invalidate_graph(&c->self);
+  return rc;
 }
 
 static int ensure_prev(stsp_ctx* c, int n) {
@@ -570,6 +592,7 @@ static FrameDesc make_desc(stsp_ctx* c) {
 // with the stream idle, and drops the graphs that captured the old lists.
 static int ensure_work(stsp_batch* bh, int work, hipStream_t s) {
   if (work <= bh->work_cap) return STSP_OK;
+  if (bh->noalloc) return STSP_ERR_TOO_LARGE;
   if (bh->d_work) {
     HIP_TRY(hipStreamSynchronize(s));
     if (bh->side.st) HIP_TRY(hipStreamSynchronize(bh->side.st));
@@ -590,7 +613,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s, in
                        int slot = 0) {
   if (n < 1 || base < 0 || slot < 0 || base + n > bh->maxf || slot >= bh->maxf || n > 64)
     return STSP_ERR_ARG;
-  if (!bh->side.st) {
+  if (!bh->side.st && !bh->noalloc) {
     HIP_TRY(hipStreamCreateWithFlags(&bh->side.st, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&bh->side.st2, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&bh->side.fork, hipEventDisableTiming));
@@ -1138,6 +1161,104 @@ static int run_single(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_o
   return STSP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// single-image BASS without allocation (stsp_bass_noalloc)
+// ---------------------------------------------------------------------------
+// Every device that runs the entry point holds its own copy of `scratch` for
+// the life of the process.  In host code `scratch` is only a stand-in for
+// the symbol: the device address of each copy comes from
+// hipGetSymbolAddress, taken once per device (with that device current) on
+// its first call, and only that address is handed to kernels and copies.
+// All calls on one device share the buffer and keep the last call's
+// contents; calls that overlap on one device race (a host mutex serialises
+// the host side, so the device work of two calls must also be ordered, e.g.
+// on one stream).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+struct NoAllocDevice {
+  std::mutex mu;
+  char* ptr = nullptr;  // this device's copy of `scratch`
+  stsp_ctx ctx;
+  int host[16];                         // read-back values (pageable)
+  FrameDesc h_fr[1];                    // batch table of one frame
+  int h_off[kOffArrays * 2];
+};
+static NoAllocDevice g_noalloc[kScratchDevices];
+
+// BASS capacity rule of bass_prologue_n (30 K0 slots) plus the growth
+// headroom of ensure_labels
+static int noalloc_cap(const stsp_config& cfg, int H, int W) {
+  HexGeom g = hex_geometry(cfg.sp_size, W, H);
+  const int nbuf = 30 * g.nx * g.ny;
+  return nbuf + nbuf / 4;
+}
+
+static size_t noalloc_tables(int cap) {
+  return 256 + ((sizeof(FrameDesc) + 255) & ~(size_t)255) + 256 + 256 +
+         ((sizeof(int) * (size_t)kWorkLists * cap + 255) & ~(size_t)255);
+}
+
+static size_t noalloc_bytes(const stsp_config& cfg, int H, int W) {
+  const int cap = noalloc_cap(cfg, H, W);
+  return ((pixel_bytes((size_t)H * W) + 255) & ~(size_t)255) + label_bytes(cap) +
+         noalloc_tables(cap);
+}
+
+static int noalloc_setup(NoAllocDevice& d) {
+  if (d.ptr) return STSP_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return STSP_ERR_HIP;
+  d.ptr = (char*)p;
+  return STSP_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// Lays the context and its batch of one out over the device's scratch copy.
+static stsp_ctx* noalloc_context(NoAllocDevice& d, const stsp_config& cfg, int H, int W,
+                                 hipStream_t s) {
+  stsp_ctx* c = &d.ctx;
+  c->cfg = cfg;
+  c->H = H;
+  c->W = W;
+  c->npix = H * W;
+  c->lut = make_lut();
+  c->noalloc = true;
+  c->timing = 0;
+  c->sweep_shape = 4;
+  c->wave_box = kWaveBoxMax;
+  c->win_h = c->win_w = 0;
+  c->nprev = 0;
+  c->host = d.host;
+  char* base = d.ptr;
+  carve_pixels(c, base);
+  base += (pixel_bytes((size_t)H * W) + 255) & ~(size_t)255;
+  const int cap = noalloc_cap(cfg, H, W);
+  (void)carve_labels(c, base, cap);  // sized by label_bytes: cannot overflow
+  base += label_bytes(cap);
+  stsp_batch* bh = &c->self;
+  bh->noalloc = true;
+  bh->use_graph = false;
+  bh->maxf = 1;
+  Arena a;
+  a.base = base;
+  a.size = noalloc_tables(cap);
+  bh->d_work_n = a.take<int>(kWorkCounters);
+  bh->d_fr = a.take<FrameDesc>(1);
+  bh->d_off = a.take<int>(kOffArrays * 2);
+  bh->d_work = a.take<int>((size_t)kWorkLists * cap);
+  bh->work_cap = cap;
+  bh->h_fr = d.h_fr;
+  bh->h_off = d.h_off;
+  k_zero_u64<<<1, 64, 0, s>>>(c->search_count, kSearchPaths);
+  return c;
+}
+
 extern "C" {
 
 void stsp_default_config(stsp_config* cfg) {
@@ -1165,6 +1286,7 @@ const char* stsp_status_string(int status) {
     case STSP_ERR_FILL: return "hole filling stopped shrinking";
     case STSP_ERR_HIP: return "HIP runtime error";
     case STSP_ERR_STATE: return "invalid stream state";
+    case STSP_ERR_TOO_LARGE: return "frame exceeds the static scratch buffer";
     default: return "unknown status";
   }
 }
@@ -1435,6 +1557,34 @@ int stsp_lab2rgb(const float* img_lab, float* img_rgb, int nimages, int npix, hi
   if (!img_lab || !img_rgb || nimages < 1 || npix < 1) return STSP_ERR_ARG;
   launch_lab_to_rgb(img_lab, img_rgb, (long long)nimages * npix, stream);
   return hipGetLastError() == hipSuccess ? STSP_OK : STSP_ERR_HIP;
+}
+
+int stsp_bass_noalloc_scratch_bytes(const stsp_config* cfg, int height, int width,
+                                    size_t* bytes) {
+  if (!cfg || !bytes || height < 2 || width < 2 || cfg->sp_size < 1) return STSP_ERR_ARG;
+  *bytes = noalloc_bytes(*cfg, height, width);
+  return STSP_OK;
+}
+
+int stsp_bass_noalloc(const stsp_config* cfg, int height, int width, const float* img,
+                      int* spix_out, stsp_params* out, int* nspix_out, hipStream_t s) {
+  if (!cfg || !img || !spix_out || !out || !nspix_out || height < 2 || width < 2 ||
+      cfg->sp_size < 1 || cfg->target_nspix > 0)
+    return STSP_ERR_ARG;
+  if ((long long)height * width >= (1ll << 23)) return STSP_ERR_ARG;
+  if (noalloc_bytes(*cfg, height, width) > kScratchBytes) return STSP_ERR_TOO_LARGE;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return STSP_ERR_ARG;
+  NoAllocDevice& d = g_noalloc[dev];
+  std::lock_guard<std::mutex> lock(d.mu);
+  int rc = noalloc_setup(d);
+  if (rc) return rc;
+  stsp_ctx* c = noalloc_context(d, *cfg, height, width, s);
+  const float* im = img;
+  rc = bass_prologue_n(&c, &im, 1, nullptr, s);
+  if (rc) return rc;
+  return run_single(c, spix_out, out, nspix_out, s);
 }
 
 int stsp_marked_video(const float* vid, const int* spix, const float* color_host, int nframes,

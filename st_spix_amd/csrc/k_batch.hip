@@ -1696,23 +1696,28 @@ __global__ void kb_add_max(Batch b) {
 }
 
 // The three BFS kernels write disjoint pixels (different labels): the few
-// long workgroup searches run on two side streams beside the wave searches.
+// long workgroup searches run on two side streams beside the wave searches
+// (all on the main stream when the batch has no side streams).
 // Expects the halves map to equal the labels: the statistics pass with
 // bounding boxes that precedes every split writes it.
 void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
   kb_reset_work<<<1, 64, 0, s>>>(b);
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  (void)hipEventRecord(side.fork, s);
-  (void)hipStreamWaitEvent(side.st, side.fork, 0);
-  (void)hipStreamWaitEvent(side.st2, side.fork, 0);
-  kb_bfs_big<<<64, 512, 0, side.st2>>>(b, cur, direction);
-  kb_bfs_block<<<256, 256, 0, side.st>>>(b, cur, direction);
+  if (side.st) {
+    (void)hipEventRecord(side.fork, s);
+    (void)hipStreamWaitEvent(side.st, side.fork, 0);
+    (void)hipStreamWaitEvent(side.st2, side.fork, 0);
+  }
+  kb_bfs_big<<<64, 512, 0, side.st ? side.st2 : s>>>(b, cur, direction);
+  kb_bfs_block<<<256, 256, 0, side.st ? side.st : s>>>(b, cur, direction);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
-  (void)hipEventRecord(side.join, side.st);
-  (void)hipEventRecord(side.join2, side.st2);
-  (void)hipStreamWaitEvent(s, side.join, 0);
-  (void)hipStreamWaitEvent(s, side.join2, 0);
+  if (side.st) {
+    (void)hipEventRecord(side.join, side.st);
+    (void)hipEventRecord(side.join2, side.st2);
+    (void)hipStreamWaitEvent(s, side.join, 0);
+    (void)hipStreamWaitEvent(s, side.join2, 0);
+  }
   kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 0);
