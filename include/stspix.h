@@ -131,6 +131,18 @@ int stsp_prop(stsp_ctx* ctx, const float* img, const float* flow, const int* spi
               stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
               int* nspix_out, int* niters_state, hipStream_t stream);
 
+/* stsp_prop plus the two intermediate maps the reference's run_prop and
+ * stream_bass return as pre_fill / post_fill (st_spix/prop/__init__.py:
+ * 282, 656): shifted_out receives the flow-shifted labels with -1 at holes
+ * and collisions (run_shift_labels, shift_and_fill.cu:165-170), filled_out the
+ * labels after hole filling and the connectivity pass, which start the
+ * refinement (shift_and_fill.cu:188-209).  Either may be NULL; each holds
+ * height*width int32 when given. */
+int stsp_prop_ex(stsp_ctx* ctx, const float* img, const float* flow, const int* spix_prev,
+                 stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
+                 int* nspix_out, int* niters_state, int* shifted_out, int* filled_out,
+                 hipStream_t stream);
+
 /* Batched (served) form: the refinement loops of several propagated frames,
  * one per context, run as shared launches (one launch covers every frame;
  * per-label work is pulled from work lists), optionally replayed from a

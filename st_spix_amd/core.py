@@ -12,6 +12,22 @@ import torch
 
 from . import _lib
 
+class ParamSet(dict):
+    """Superpixel parameters: a dict of CUDA tensors named after the
+    reference's SuperpixelParams fields (st_spix/csrc/bist/structs.h:13-48),
+    also readable and writable as attributes (``params.mu_app``), as the
+    reference's callers use them (st_spix/prop/__init__.py:175, 300)."""
+
+    def __getattr__(self, name):
+        try:
+            return self[name]
+        except KeyError:
+            raise AttributeError(name) from None
+
+    def __setattr__(self, name, value):
+        self[name] = value
+
+
 Params = Dict[str, torch.Tensor]
 
 _DTYPES = {"float32": torch.float32, "float64": torch.float64, "int32": torch.int32}
@@ -133,7 +149,7 @@ def alloc_params(cap: int, device, zero: bool = True) -> Params:
     """Parameter arrays of `cap` slots.  The export writes every field of
     slots [0, nspix) and results are trimmed to nspix, so callers on the
     served path skip the zero fill (zero=False)."""
-    out = {}
+    out = ParamSet()
     make = torch.zeros if zero else torch.empty
     for name, dt, w in _lib.PARAM_FIELDS:
         shape = (cap, w) if w > 1 else (cap,)
@@ -155,7 +171,7 @@ def params_struct(p: Params, cap: Optional[int] = None) -> _lib.ParamsStruct:
 
 
 def trim(p: Params, n: int) -> Params:
-    return {k: v[:n] for k, v in p.items()}
+    return ParamSet((k, v[:n]) for k, v in p.items())
 
 
 def _stream() -> ctypes.c_void_p:
@@ -204,10 +220,12 @@ def check_prop_inputs(ctx: Optional["Context"], img: torch.Tensor, flow: torch.T
 
 def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
                params_prev: Params, cfg: _lib.Config,
-               ctx: Optional[Context] = None) -> Tuple[torch.Tensor, Params, int]:
+               ctx: Optional[Context] = None, maps: bool = False):
     """One propagated frame.  ``params_prev['prior_counts']`` and
     ``params_prev['invalid_counts']`` are updated in place, as the reference's
-    shift_and_fill does.  Returns (spix, params, adaptive_niters)."""
+    shift_and_fill does.  Returns (spix, params, adaptive_niters); with
+    ``maps`` also the shifted labels (-1 at holes) and the filled labels that
+    start the refinement (stsp_prop_ex)."""
     check_prop_inputs(ctx, img, flow, spix_prev)
     H, W = img.shape[:2]
     nprev = int(params_prev["ids"].shape[0])
@@ -220,13 +238,21 @@ def prop_frame(img: torch.Tensor, flow: torch.Tensor, spix_prev: torch.Tensor,
     spix = torch.empty((H, W), dtype=torch.int32, device=img.device)
     n = ctypes.c_int(0)
     ns = ctypes.c_int(0)
+    shifted = filled = None
+    if maps:
+        shifted = torch.empty((H, W), dtype=torch.int32, device=img.device)
+        filled = torch.empty((H, W), dtype=torch.int32, device=img.device)
     with torch.cuda.device(img.device):
-        rc = _lib.lib().stsp_prop(ctx.handle, img.data_ptr(), flow.data_ptr(),
-                                  spix_prev.data_ptr(),
-                                  ctypes.byref(params_struct(params_prev, nprev)), nprev,
-                                  spix.data_ptr(), ctypes.byref(params_struct(out, cap)),
-                                  ctypes.byref(n), ctypes.byref(ns), _stream())
-    _lib.check(rc, "stsp_prop")
+        rc = _lib.lib().stsp_prop_ex(ctx.handle, img.data_ptr(), flow.data_ptr(),
+                                     spix_prev.data_ptr(),
+                                     ctypes.byref(params_struct(params_prev, nprev)), nprev,
+                                     spix.data_ptr(), ctypes.byref(params_struct(out, cap)),
+                                     ctypes.byref(n), ctypes.byref(ns),
+                                     shifted.data_ptr() if maps else None,
+                                     filled.data_ptr() if maps else None, _stream())
+    _lib.check(rc, "stsp_prop_ex")
+    if maps:
+        return spix, trim(out, n.value), ns.value, shifted, filled
     return spix, trim(out, n.value), ns.value
 
 

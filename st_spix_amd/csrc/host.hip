@@ -1009,10 +1009,14 @@ static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStre
 // shift_and_fill (shift_and_fill.cu:108-339) + run_prop set-up
 // (prop.cu:677-745) for frames cs[0..n): every stage is one launch over all
 // frames (the ProFrame table), and the host waits only for the hole counts.
+// shifted_out / filled_out (may be null, or hold null entries) receive
+// copies of each frame's shifted map (holes -1) and of the filled map that
+// starts the refinement.
 static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* const* imgs,
                            const float* const* flows, const int* const* spix_prev,
                            stsp_params* prevs, const int* nprev, int* niters_state,
-                           hipStream_t s) {
+                           hipStream_t s, int* const* shifted_out = nullptr,
+                           int* const* filled_out = nullptr) {
   if (n > bh->maxf || n > kProMax) return STSP_ERR_ARG;
   int* off = bh->h_pro_off;
   for (int k = 0; k < kProKinds; k++) off[k * (kProMax + 1)] = 0;
@@ -1075,6 +1079,13 @@ static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* co
   rc = fill_loop_pro(bh, p, total, s);
   if (rc) return rc;
   pro_finish(p, total, s);
+  for (int i = 0; i < n; i++) {
+    const size_t bytes = sizeof(int) * (size_t)cs[i]->npix;
+    if (shifted_out && shifted_out[i])
+      HIP_TRY(hipMemcpyAsync(shifted_out[i], cs[i]->shifted, bytes, hipMemcpyDeviceToDevice, s));
+    if (filled_out && filled_out[i])
+      HIP_TRY(hipMemcpyAsync(filled_out[i], cs[i]->seg, bytes, hipMemcpyDeviceToDevice, s));
+  }
   HIP_TRY(hipMemcpyAsync(bh->h_negs, p.negs, sizeof(int) * n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   for (int i = 0; i < n; i++) {
@@ -1392,16 +1403,24 @@ int stsp_bass(stsp_ctx* c, const float* img, int* spix_out, stsp_params* out, in
   return run_single(c, spix_out, out, nspix_out, s);
 }
 
-int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
-              stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
-              int* nspix_out, int* niters_state, hipStream_t s) {
+int stsp_prop_ex(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
+                 stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
+                 int* nspix_out, int* niters_state, int* shifted_out, int* filled_out,
+                 hipStream_t s) {
   if (!c || !img || !flow || !spix_prev || !prev || !spix_out || !out || !nspix_out ||
       nspix_prev < 1)
     return STSP_ERR_ARG;
   int rc = prop_prologue_n(&c->self, &c, 1, &img, &flow, &spix_prev, prev, &nspix_prev,
-                           niters_state, s);
+                           niters_state, s, &shifted_out, &filled_out);
   if (rc) return rc;
   return run_single(c, spix_out, out, nspix_out, s);
+}
+
+int stsp_prop(stsp_ctx* c, const float* img, const float* flow, const int* spix_prev,
+              stsp_params* prev, int nspix_prev, int* spix_out, stsp_params* out,
+              int* nspix_out, int* niters_state, hipStream_t s) {
+  return stsp_prop_ex(c, img, flow, spix_prev, prev, nspix_prev, spix_out, out, nspix_out,
+                      niters_state, nullptr, nullptr, s);
 }
 
 int stsp_batch_create(int max_frames, stsp_batch** out) {
