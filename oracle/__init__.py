@@ -211,3 +211,100 @@ def bist(vid, flows, cfg: Cfg, video_mode: bool = True):
                                               niters_state)
         out.append(spix)
     return np.stack(out)
+
+
+# ---------------------------------------------------------------- unit steps
+def calc_joint(img_px, x, y, mu_app, mu_shape, isig, logdet, count, sigma2_app, neigh_neq,
+               potts):
+    """calc_joint (update_seg.cu:194-259) of one pixel against one label."""
+    L = lib()
+    L.orc_calc_joint.restype = ctypes.c_float
+    a = [np.ascontiguousarray(v, t) for v, t in ((img_px, np.float32), (mu_app, np.float32),
+                                                 (mu_shape, np.float64), (isig, np.float64))]
+    return float(L.orc_calc_joint(_p(a[0]), int(x), int(y), _p(a[1]), _p(a[2]), _p(a[3]),
+                                  ctypes.c_double(logdet), int(count),
+                                  ctypes.c_float(sigma2_app), ctypes.c_float(neigh_neq),
+                                  ctypes.c_float(potts)))
+
+
+def nig_marginal(sigma2_app, n, bn):
+    L = lib()
+    L.orc_nig_marginal.restype = ctypes.c_double
+    b = np.ascontiguousarray(bn, np.float64)
+    return float(L.orc_nig_marginal(ctypes.c_float(sigma2_app), ctypes.c_double(n), _p(b)))
+
+
+def calc_simple_update(stat, prop, prior_count, prior_icov, sample_sigma_shape, sp_size):
+    stat = np.ascontiguousarray(stat, np.int64)
+    mu_app = np.zeros(3, np.float32)
+    mu_shape = np.zeros(2)
+    isig = np.zeros(3)
+    logdet = ctypes.c_double(0)
+    lib().orc_calc_simple_update(_p(stat), int(prop), ctypes.c_float(prior_count),
+                                 _p(np.ascontiguousarray(prior_icov, np.float64)),
+                                 _p(np.ascontiguousarray(sample_sigma_shape, np.float64)),
+                                 int(sp_size), _p(mu_app), _p(mu_shape), _p(isig),
+                                 ctypes.byref(logdet))
+    return mu_app, mu_shape, isig, logdet.value
+
+
+def invert_cov(cov):
+    out = np.zeros(3)
+    lib().orc_invert_cov(_p(np.ascontiguousarray(cov, np.float64)), _p(out))
+    return out
+
+
+def flow_pool(flow, seg, nspix):
+    flow = np.ascontiguousarray(flow, np.float32)
+    seg = np.ascontiguousarray(seg, np.int32)
+    ds = np.zeros((nspix, 2), np.float32)
+    cnt = np.zeros(nspix, np.int32)
+    lib().orc_flow_pool(_p(flow), _p(seg), seg.size, int(nspix), _p(ds), _p(cnt))
+    return ds, cnt
+
+
+def split_eval(img, seg, K, direction, sigma2_app, split_alpha=0.0, prop=None, shifted=None,
+               prop_mode=0):
+    """Split proposal and Hastings ratios of labels 0..K-1 (all valid)."""
+    img = np.ascontiguousarray(img, np.float32)
+    seg = np.ascontiguousarray(seg, np.int32)
+    H, W = seg.shape
+    pr = np.ascontiguousarray(prop if prop is not None else np.zeros(K), np.int32)
+    sh = np.ascontiguousarray(shifted if shifted is not None else seg, np.int32)
+    halves = np.zeros((H, W), np.int32)
+    hasting = np.zeros(K)
+    accept = np.zeros(K, np.int32)
+    select = np.zeros(K, np.int32)
+    lib().orc_split_eval(_p(img), _p(seg), _p(sh), H, W, int(K), _p(pr), int(direction),
+                         ctypes.c_float(sigma2_app), ctypes.c_float(split_alpha),
+                         int(prop_mode), _p(halves), _p(hasting), _p(accept), _p(select))
+    return halves, hasting, accept, select
+
+
+def merge_eval(img, seg, K, direction, sigma2_app, alpha_hastings, merge_alpha=0.0, prop=None,
+               prop_mode=0):
+    img = np.ascontiguousarray(img, np.float32)
+    seg = np.ascontiguousarray(seg, np.int32)
+    H, W = seg.shape
+    pr = np.ascontiguousarray(prop if prop is not None else np.zeros(K), np.int32)
+    cand = np.zeros(K, np.int32)
+    hasting = np.zeros(K)
+    winner = np.zeros(K, np.int32)
+    lib().orc_merge_eval(_p(img), _p(seg), H, W, int(K), _p(pr), int(direction),
+                         ctypes.c_float(sigma2_app), ctypes.c_float(alpha_hastings),
+                         ctypes.c_float(merge_alpha), int(prop_mode), _p(cand), _p(hasting),
+                         _p(winner))
+    return cand, hasting, winner
+
+
+def relabel_search(mu_app, living, prior_mu_app, nspix_prev, thresh_relabel, thresh_new,
+                   nbuf):
+    mu_app = np.ascontiguousarray(mu_app, np.float32)
+    living = np.ascontiguousarray(living, np.uint8)
+    pm = np.ascontiguousarray(prior_mu_app, np.float32)
+    n = len(living)
+    target = np.zeros(n, np.int32)
+    nxt = lib().orc_relabel_search(_p(mu_app), _p(living), n - 1, _p(pm), int(nspix_prev),
+                                   ctypes.c_float(thresh_relabel), ctypes.c_float(thresh_new),
+                                   int(nbuf), _p(target))
+    return target, nxt

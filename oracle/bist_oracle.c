@@ -669,17 +669,11 @@ static void sm_accumulate(const float* img, const int* lab, const int* shifted, 
 
 /* CudaCalcSplitCandidate_orig / _p: split_merge_orig.cu:202-420,
  * split_merge_prop.cu:184-395.  prop_mode selects update_split_flag_p. */
-static int split_step(const float* img, int* seg, const int* shifted, osp* P,
-                      const ostat* st, int nbuf, int max_spix, int direction,
-                      const nig_consts* nc, float split_alpha, int sp_size, int prop_mode,
-                      int width, int height, sm_work* w) {
-  if (max_spix > nbuf / 2) return max_spix;
-  int npix = width * height;
-  sm_clear(w, nbuf);
-  split_halves(seg, P, nbuf, max_spix, direction, width, height, w);
-  sm_accumulate(img, w->halves, prop_mode ? shifted : NULL, npix, width, w);
-  int nacc = 0;
-  int* acc = (int*)calloc((size_t)nbuf, sizeof(int));
+/* calc_split_stats_step0/1(_p) + the Hastings ratio of update_split_flag(_p):
+ * hasting and select per label, acc[k] = 1 where the split is accepted
+ * (split_merge_prop.cu:2861-3070, split_merge.cu:862-962) */
+static void split_eval(osp* P, const ostat* st, int nbuf, int max_spix, const nig_consts* nc,
+                       float split_alpha, int prop_mode, sm_work* w, int* acc) {
   for (int k = 0; k < nbuf; k++) {
     if (P[k].valid == 0) continue;
     int s = k + (max_spix + 1);
@@ -711,11 +705,22 @@ static int split_step(const float* img, int* seg, const int* shifted, osp* P,
       h = h - 2.0 * (1.0 - iperc);
     }
     w->sm[k].hasting = h;
-    if (h > -2.0) {
-      acc[k] = 1;
-      nacc++;
-    }
+    if (h > -2.0) acc[k] = 1;
   }
+}
+
+static int split_step(const float* img, int* seg, const int* shifted, osp* P,
+                      const ostat* st, int nbuf, int max_spix, int direction,
+                      const nig_consts* nc, float split_alpha, int sp_size, int prop_mode,
+                      int width, int height, sm_work* w) {
+  if (max_spix > nbuf / 2) return max_spix;
+  int npix = width * height;
+  sm_clear(w, nbuf);
+  split_halves(seg, P, nbuf, max_spix, direction, width, height, w);
+  sm_accumulate(img, w->halves, prop_mode ? shifted : NULL, npix, width, w);
+  int* acc = (int*)calloc((size_t)nbuf, sizeof(int));
+  split_eval(P, st, nbuf, max_spix, nc, split_alpha, prop_mode, w, acc);
+  (void)sp_size;
   /* rule R4: new ids in ascending k */
   int next = max_spix;
   for (int k = 0; k < nbuf; k++) {
@@ -775,10 +780,13 @@ static int split_step(const float* img, int* seg, const int* shifted, osp* P,
 
 /* CudaCalcMergeCandidate_orig / _p: split_merge_orig.cu:79-199,
  * split_merge_prop.cu:87-181 */
-static void merge_step(const float* img, int* seg, const unsigned char* border, osp* P,
-                       const ostat* st, int nbuf, int direction, const nig_consts* nc,
-                       double log_alpha_f, float merge_alpha, int prop_mode, int width,
-                       int height, sm_work* w) {
+/* candidates, sums, calc_merge_stats_step0/1/2(_p) and update_merge_flag(_p):
+ * pairs[2k+1] = k's candidate f, sm[k].hasting, win[f] = the accepted k that
+ * takes f (rule R2) or -1 */
+static void merge_eval(const float* img, const int* seg, const unsigned char* border,
+                       const osp* P, const ostat* st, int nbuf, int direction,
+                       const nig_consts* nc, double log_alpha_f, float merge_alpha,
+                       int prop_mode, int width, int height, sm_work* w, int* win) {
   int npix = width * height;
   sm_clear(w, nbuf);
   /* calc_merge_candidate(_p), split_merge.cu:719-755, split_merge_prop.cu:1808-1845 */
@@ -825,7 +833,6 @@ static void merge_step(const float* img, int* seg, const unsigned char* border, 
     if (P[k].valid == 0) continue;
     mk[k] = nig_marginal(nc, (double)P[k].count, w->sm[k].b_n); /* b_n = 0 unless step0 ran */
   }
-  int* win = (int*)malloc(sizeof(int) * (size_t)nbuf);
   for (int k = 0; k < nbuf; k++) win[k] = -1;
   for (int k = 0; k < nbuf; k++) {
     if (P[k].valid == 0) continue;
@@ -839,6 +846,17 @@ static void merge_step(const float* img, int* seg, const unsigned char* border, 
     /* update_merge_flag(_p), rule R2 */
     if (h > -2.0 && k > win[f]) win[f] = k;
   }
+  free(mk);
+}
+
+static void merge_step(const float* img, int* seg, const unsigned char* border, osp* P,
+                       const ostat* st, int nbuf, int direction, const nig_consts* nc,
+                       double log_alpha_f, float merge_alpha, int prop_mode, int width,
+                       int height, sm_work* w) {
+  int npix = width * height;
+  int* win = (int*)malloc(sizeof(int) * (size_t)nbuf);
+  merge_eval(img, seg, border, P, st, nbuf, direction, nc, log_alpha_f, merge_alpha, prop_mode,
+             width, height, w, win);
   for (int f = 0; f < nbuf; f++)
     if (win[f] >= 0) {
       w->sm[win[f]].merge = 1;
@@ -889,7 +907,6 @@ static void merge_step(const float* img, int* seg, const unsigned char* border, 
   free(pairs0);
   free(valid0);
   free(pc0);
-  free(mk);
   free(win);
 }
 
@@ -1355,14 +1372,12 @@ void orc_invalidate_disconnected(int* seg, int height, int width, int nspix) {
 }
 
 /* relabel_spix, relabel.cu:279-409 (rules R7, R8) */
-static int relabel_step(int* seg, int npix, osp* P, int nbuf, const orc_params* prev,
-                        float thresh_replace, float thresh_new, int nspix_prev,
-                        int max_spix) {
+/* find_most_similar_spix + mark_for_relabel (relabel.cu:125-276, rules R7,
+ * R8): target[s] for every label s < max_spix + 1; returns the new max label */
+static int relabel_search(const float* mu_app, int mu_stride, const unsigned char* living,
+                          int max_spix, const float* prior_mu_app, int nspix_prev,
+                          float thresh_replace, float thresh_new, int nbuf, int* target) {
   int nspix = max_spix + 1;
-  unsigned char* living = (unsigned char*)calloc((size_t)nspix, 1);
-  for (int i = 0; i < npix; i++)
-    if (seg[i] >= 0 && seg[i] < nspix) living[seg[i]] = 1;
-  int* target = (int*)malloc(sizeof(int) * (size_t)nspix);
   for (int s = 0; s < nspix; s++) target[s] = -1;
   int next = max_spix;
   for (int s = 0; s < nspix; s++) {
@@ -1374,9 +1389,10 @@ static int relabel_step(int* seg, int npix, osp* P, int nbuf, const orc_params* 
     for (int c = 0; c < nspix_prev; c++) {
       int is_same = (c == s);
       if (living[c] && !is_same) continue;
-      float dx = P[s].mu_app[0] - prev->prior_mu_app[3 * c + 0];
-      float dy = P[s].mu_app[1] - prev->prior_mu_app[3 * c + 1];
-      float dz = P[s].mu_app[2] - prev->prior_mu_app[3 * c + 2];
+      const float* m = mu_app + (size_t)mu_stride * s;
+      float dx = m[0] - prior_mu_app[3 * c + 0];
+      float dy = m[1] - prior_mu_app[3 * c + 1];
+      float dz = m[2] - prior_mu_app[3 * c + 2];
       float delta = (dx * dx + dy * dy + dz * dz);
       if (is_same) ss = delta;
       if (delta < best) {
@@ -1392,6 +1408,22 @@ static int relabel_step(int* seg, int npix, osp* P, int nbuf, const orc_params* 
       if (next + 1 < nbuf) target[s] = ++next;
     }
   }
+  return next;
+}
+
+static int relabel_step(int* seg, int npix, osp* P, int nbuf, const orc_params* prev,
+                        float thresh_replace, float thresh_new, int nspix_prev,
+                        int max_spix) {
+  int nspix = max_spix + 1;
+  unsigned char* living = (unsigned char*)calloc((size_t)nspix, 1);
+  for (int i = 0; i < npix; i++)
+    if (seg[i] >= 0 && seg[i] < nspix) living[seg[i]] = 1;
+  int* target = (int*)malloc(sizeof(int) * (size_t)nspix);
+  float* mu = (float*)malloc(sizeof(float) * 3 * (size_t)nspix);
+  for (int k = 0; k < nspix; k++) memcpy(mu + 3 * k, P[k].mu_app, sizeof(float) * 3);
+  int next = relabel_search(mu, 3, living, max_spix, prev->prior_mu_app, nspix_prev,
+                            thresh_replace, thresh_new, nbuf, target);
+  free(mu);
   /* relabel_spix_kernel, relabel.cu:85-120: parameters from the largest source */
   for (int s = 0; s < nspix; s++) {
     int n = target[s];
@@ -1593,4 +1625,157 @@ int orc_prop(const float* img, const float* flow, int height, int width,
   free(counts_s);
   free(counts_f);
   return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* unit entry points: single steps of the restatement, for the tests   */
+/* that pin it against closed forms and brute force                    */
+/* ------------------------------------------------------------------ */
+float orc_calc_joint(const float* imgC, int x, int y, const float* mu_app, const double* mu_shape,
+                     const double* isig, double logdet, int count, float sigma2_app,
+                     float neigh_neq, float potts) {
+  sweep_params sp = {mu_app, mu_shape, isig, &logdet, &count};
+  float best = -INFINITY;
+  int lab = -1;
+  calc_joint(imgC, x, y, &sp, 0, sigma2_app, neigh_neq, potts, &best, &lab);
+  return best;
+}
+
+double orc_nig_marginal(float sigma2_app, double n, const double* bn) {
+  nig_consts nc = make_nig(sigma2_app);
+  return nig_marginal(&nc, n, bn);
+}
+
+void orc_calc_simple_update(const int64_t* stat, int prop, float prior_count,
+                            const double* prior_icov, const double* sample_sigma_shape,
+                            int sp_size, float* mu_app, double* mu_shape, double* isig,
+                            double* logdet) {
+  osp p;
+  memset(&p, 0, sizeof(p));
+  ostat st;
+  memset(&st, 0, sizeof(st));
+  st.count = (int)stat[0];
+  for (int c = 0; c < 3; c++) st.S[c] = stat[1 + c];
+  st.X = stat[4];
+  st.Y = stat[5];
+  st.XX = stat[6];
+  st.XY = stat[7];
+  st.YY = stat[8];
+  p.count = st.count;
+  p.valid = 1;
+  p.prop = prop;
+  p.prior_count = prior_count;
+  memcpy(p.prior_icov, prior_icov, sizeof(double) * 3);
+  memcpy(p.sample_sigma_shape, sample_sigma_shape, sizeof(double) * 3);
+  calc_simple_update(&p, &st, sp_size);
+  memcpy(mu_app, p.mu_app, sizeof(float) * 3);
+  memcpy(mu_shape, p.mu_shape, sizeof(double) * 2);
+  memcpy(isig, p.sigma_shape, sizeof(double) * 3);
+  *logdet = p.logdet_sigma_shape;
+}
+
+void orc_invert_cov(const double* cov, double* out) {
+  double pr_det = cov[0] * cov[2] - cov[1] * cov[1];
+  out[0] = cov[2] / pr_det;
+  out[1] = cov[1] / pr_det;
+  out[2] = cov[0] / pr_det;
+}
+
+/* the per-label mean flow of orc_prop (run_sp_downsample, rule R1) */
+void orc_flow_pool(const float* flow, const int* seg, int npix, int nspix, float* flow_ds,
+                   int* counts) {
+  int64_t* fsum = (int64_t*)calloc((size_t)2 * nspix, sizeof(int64_t));
+  memset(counts, 0, sizeof(int) * (size_t)nspix);
+  for (int t = 0; t < npix; t++) {
+    int L = seg[t];
+    if (L < 0 || L >= nspix) continue;
+    fsum[2 * L] += (int64_t)llrint((double)flow[2 * t] * FX_FLOW);
+    fsum[2 * L + 1] += (int64_t)llrint((double)flow[2 * t + 1] * FX_FLOW);
+    counts[L]++;
+  }
+  for (int k = 0; k < nspix; k++)
+    for (int c = 0; c < 2; c++) {
+      double sum = (double)fsum[2 * k + c] * (1.0 / FX_FLOW);
+      flow_ds[2 * k + c] = counts[k] > 0 ? (float)(sum / (1. * counts[k])) : 0.0f;
+    }
+  free(fsum);
+}
+
+/* per-label working state of labels 0..K-1 of seg, as update_params leaves it */
+static osp* unit_state(const float* img, const int* seg, int npix, int width, int K,
+                       const int* prop, int nbuf, ostat** st_out) {
+  osp* P = (osp*)calloc((size_t)nbuf, sizeof(osp));
+  ostat* st = (ostat*)calloc((size_t)nbuf, sizeof(ostat));
+  stats_accumulate(img, seg, npix, width, st, nbuf, NULL);
+  for (int k = 0; k < K; k++) {
+    P[k].count = st[k].count;
+    P[k].valid = st[k].count > 0;
+    P[k].prop = prop ? prop[k] : 0;
+    if (st[k].count > 0) {
+      P[k].mu_shape[0] = (double)st[k].X / st[k].count;
+      P[k].mu_shape[1] = (double)st[k].Y / st[k].count;
+    }
+  }
+  *st_out = st;
+  return P;
+}
+
+int orc_split_eval(const float* img, const int* seg, const int* shifted, int height, int width,
+                   int K, const int* prop, int direction, float sigma2_app, float split_alpha,
+                   int prop_mode, int* halves, double* hasting, int* accept, int* select) {
+  int npix = width * height, nbuf = 2 * K + 2, max_spix = K - 1;
+  ostat* st;
+  osp* P = unit_state(img, seg, npix, width, K, prop, nbuf, &st);
+  sm_work w = sm_alloc(npix, nbuf);
+  nig_consts nc = make_nig(sigma2_app);
+  sm_clear(&w, nbuf);
+  split_halves(seg, P, nbuf, max_spix, direction, width, height, &w);
+  sm_accumulate(img, w.halves, prop_mode ? shifted : NULL, npix, width, &w);
+  int* acc = (int*)calloc((size_t)nbuf, sizeof(int));
+  split_eval(P, st, nbuf, max_spix, &nc, split_alpha, prop_mode, &w, acc);
+  memcpy(halves, w.halves, sizeof(int) * (size_t)npix);
+  for (int k = 0; k < K; k++) {
+    hasting[k] = w.sm[k].hasting;
+    accept[k] = acc[k];
+    select[k] = w.sm[k].select;
+  }
+  free(acc);
+  sm_free(&w);
+  free(st);
+  free(P);
+  return 0;
+}
+
+int orc_merge_eval(const float* img, const int* seg, int height, int width, int K,
+                   const int* prop, int direction, float sigma2_app, float alpha_hastings,
+                   float merge_alpha, int prop_mode, int* cand, double* hasting, int* winner) {
+  int npix = width * height, nbuf = 2 * K + 2;
+  ostat* st;
+  osp* P = unit_state(img, seg, npix, width, K, prop, nbuf, &st);
+  sm_work w = sm_alloc(npix, nbuf);
+  nig_consts nc = make_nig(sigma2_app);
+  unsigned char* border = (unsigned char*)malloc((size_t)npix);
+  find_border_pixels(seg, border, height, width);
+  int* win = (int*)malloc(sizeof(int) * (size_t)nbuf);
+  merge_eval(img, seg, border, P, st, nbuf, direction, &nc,
+             stsp_det_log((double)expf(alpha_hastings)), merge_alpha, prop_mode, width, height,
+             &w, win);
+  for (int k = 0; k < K; k++) {
+    cand[k] = w.pairs[2 * k + 1];
+    hasting[k] = w.sm[k].hasting;
+    winner[k] = win[k];
+  }
+  free(win);
+  free(border);
+  sm_free(&w);
+  free(st);
+  free(P);
+  return 0;
+}
+
+int orc_relabel_search(const float* mu_app, const unsigned char* living, int max_spix,
+                       const float* prior_mu_app, int nspix_prev, float thresh_relabel,
+                       float thresh_new, int nbuf, int* target) {
+  return relabel_search(mu_app, 3, living, max_spix, prior_mu_app, nspix_prev, thresh_relabel,
+                        thresh_new, nbuf, target);
 }

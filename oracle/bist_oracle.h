@@ -93,6 +93,39 @@ int orc_fill_missing(int* seg, const double* centers, int height, int width,
                      int break_iter);
 void orc_invalidate_disconnected(int* seg, int height, int width, int nspix);
 
+/* Single steps of the restatement, exposed so the tests can pin each one
+ * against closed forms, library routines and brute force. */
+/* calc_joint (update_seg.cu:194-259) of one pixel for one candidate label */
+float orc_calc_joint(const float* imgC, int x, int y, const float* mu_app,
+                     const double* mu_shape, const double* isig, double logdet, int count,
+                     float sigma2_app, float neigh_neq, float potts);
+/* sum over channels of the NIG marginal terms (split_merge_prop.cu:2934-3070) */
+double orc_nig_marginal(float sigma2_app, double n, const double* bn);
+/* calc_simple_update (update_params.cu:220-381) for one label; stat =
+ * count, S_r, S_g, S_b (2^-32 fixed point), X, Y, XX, XY, YY */
+void orc_calc_simple_update(const int64_t* stat, int prop, float prior_count,
+                            const double* prior_icov, const double* sample_sigma_shape,
+                            int sp_size, float* mu_app, double* mu_shape, double* isig,
+                            double* logdet);
+/* invert_cov (sparams_io.cu:504-520) */
+void orc_invert_cov(const double* cov, double* out);
+/* run_sp_downsample (sp_pooling.cu:27-71) as orc_prop computes it */
+void orc_flow_pool(const float* flow, const int* seg, int npix, int nspix, float* flow_ds,
+                   int* counts);
+/* split proposal and Hastings ratios for labels 0..K-1 of seg (all valid,
+ * max_spix = K-1): halves map, hasting, accept and select per label */
+int orc_split_eval(const float* img, const int* seg, const int* shifted, int height, int width,
+                   int K, const int* prop, int direction, float sigma2_app, float split_alpha,
+                   int prop_mode, int* halves, double* hasting, int* accept, int* select);
+/* merge candidates, Hastings ratios and the winner of each target f */
+int orc_merge_eval(const float* img, const int* seg, int height, int width, int K,
+                   const int* prop, int direction, float sigma2_app, float alpha_hastings,
+                   float merge_alpha, int prop_mode, int* cand, double* hasting, int* winner);
+/* relabel targets of labels 0..max_spix; returns the new max label */
+int orc_relabel_search(const float* mu_app, const unsigned char* living, int max_spix,
+                       const float* prior_mu_app, int nspix_prev, float thresh_relabel,
+                       float thresh_new, int nbuf, int* target);
+
 #ifdef __cplusplus
 }
 #endif
