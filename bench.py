@@ -1,14 +1,31 @@
 #!/usr/bin/env python
 """Streaming BIST benchmark: frames/s and p50/p99 frame latency over 16
 mixed-resolution streams (4 each at 360x640, 480x854, 720x1280, 1080x1920),
-served (batched across streams) versus the plain one-frame-at-a-time loop.
+served (batched across streams, continuous batching) versus the plain
+one-frame-at-a-time loop.
 
-A step is one tick: every stream advances by one propagated frame (flow
-shift + hole fill + connectivity + 100 refinement iterations of 4 boundary
-sweeps with split/merge every 4 iterations), on synthetic Voronoi video with
-affine region motion (inputs generated on the GPU, resident in HBM before the
-timed region).  Frame 0 of every stream (BASS, 100 iterations) is part of the
-untimed warm-up.
+Inputs are synthetic Voronoi video with per-region affine motion (the
+config-1 motion model: 120 regions, |v| <= 4 px/frame, ground-truth flow
+plus 0.5 px noise), generated on the GPU and resident in HBM before any
+timed region.  Every propagated frame runs flow shift, hole fill,
+connectivity and 100 refinement iterations of 4 boundary sweeps with
+split/merge every 4 iterations.
+
+Measurements:
+  * closed loop (the JSON ``value``): a step is one tick in which every
+    stream submits one propagated frame at once and the server drains them;
+    ``steps`` ticks are timed after ``warmup`` ticks (frame 0 of every
+    stream, BASS, is warm-up).  This is the served capacity.
+  * open loop (the JSON ``p50_ms`` / ``p99_ms``): config 3's arrival
+    process, each stream a Poisson process of 30 frames/s, frames submitted
+    at their arrival times from the main thread while two executors run
+    batches; latency is arrival to completion of each propagated frame.
+  * the plain loop: the same ticks, one stsp_prop call per frame.
+  * the fused sweep kernel against the HBM roofline, and the CPU oracle.
+
+The open loop runs in a child process (this script with --open-loop-child),
+which generates the same streams itself; its result is merged into the
+JSON line.
 """
 from __future__ import annotations
 
@@ -40,6 +57,13 @@ def parse():
     ap.add_argument("--sp-size", type=int, default=25)
     ap.add_argument("--sweep-shape", type=int, default=4,
                     help="fused sweep launch shape (index into core.SWEEP_SHAPES)")
+    ap.add_argument("--open-frames", type=int, default=60,
+                    help="frames per stream of the open-loop (Poisson arrival) run")
+    ap.add_argument("--fps", type=float, default=30.0, help="open-loop arrival rate per stream")
+    ap.add_argument("--slots", type=int, default=2, help="batch executors of the open loop")
+    ap.add_argument("--no-open", action="store_true")
+    ap.add_argument("--open-loop-child", action="store_true",
+                    help="run only the open loop and print its result (used by the parent)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay the refinement as a hipGraph")
@@ -64,9 +88,12 @@ def pct(xs, q):
     return xs[k]
 
 
-def make_streams(args, torch, synth):
+def ms(x):
+    return round(x * 1e3, 3)
+
+
+def make_streams(args, torch, synth, T):
     streams = []
-    T = args.warmup + args.steps + 1
     for ri, (H, W) in enumerate(RESOLUTIONS):
         for j in range(args.streams_per_res):
             seed = 1000 * ri + j
@@ -76,19 +103,20 @@ def make_streams(args, torch, synth):
     return streams
 
 
+def stream_cfg(core, args):
+    return core.make_config(niters=args.iters, sp_size=args.sp_size, potts=0.5,
+                            prop_niters=args.iters)
+
+
 def run_plain(args, streams, torch, core):
-    cfg0 = core.make_config(niters=args.iters, sp_size=args.sp_size, potts=0.5)
-    cfg = core.make_config(niters=args.iters, sp_size=args.sp_size, potts=0.5,
-                           prop_niters=args.iters)
+    cfg = stream_cfg(core, args)
     core.set_sweep_shape(args.sweep_shape)
     for s in streams:
-        s["spix"], s["params"] = core.bass_frame(s["vid"][0], cfg0)
+        s["spix"], s["params"] = core.bass_frame(s["vid"][0], cfg)
     for t in range(1, args.warmup + 1):
         for s in streams:
             s["spix"], s["params"], _ = core.prop_frame(s["vid"][t], s["flows"][t - 1], s["spix"],
                                                         s["params"], cfg)
-    for s in streams:
-        core.get_context(s["H"], s["W"], cfg, s["vid"].device).enable_timing(True)
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -99,36 +127,17 @@ def run_plain(args, streams, torch, core):
                                                         s["params"], cfg)
             lat.append(time.perf_counter() - a)
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    # dominant kernel: the fused boundary sweep, timed with HIP events
-    sweep_ms, sweep_bytes, launches = 0.0, 0.0, 0
-    seen = set()
-    for s in streams:
-        key = (s["H"], s["W"])
-        if key in seen:
-            continue
-        seen.add(key)
-        ctx = core.get_context(s["H"], s["W"], cfg, s["vid"].device)
-        ms, n = ctx.kernel_time(0)
-        sweep_ms += ms
-        launches += n
-        sweep_bytes += n * BYTES_PER_PX_SWEEP * SWEEPS_PER_ITER * s["H"] * s["W"]
-        ctx.enable_timing(False)
-    return {"elapsed": elapsed, "lat": lat, "sweep_ms": sweep_ms, "sweep_bytes": sweep_bytes,
-            "sweep_launches": launches}
+    return {"elapsed": time.perf_counter() - t0, "lat": lat}
 
 
-def run_served(args, streams, torch, core):
-    """Continuous batching: every step takes one pending frame of each stream
-    and refines them in shared launches (stsp_prop_batch)."""
+def run_closed(args, streams, torch, core, timing=True):
+    """Closed loop: each tick submits one frame of every stream at once; the
+    server drains them (one batch of all streams per tick)."""
     from st_spix_amd import serve
     core.set_sweep_shape(args.sweep_shape)
-    srv = serve.Server(hbm_budget_gb=64, max_batch=len(streams), use_graph=bool(args.graph))
-    sids = []
-    for s in streams:
-        cfg = core.make_config(niters=args.iters, sp_size=args.sp_size, potts=0.5,
-                               prop_niters=args.iters)
-        sids.append(srv.open_stream(s["H"], s["W"], cfg))
+    srv = serve.Server(hbm_budget_gb=64, max_batch=len(streams), use_graph=bool(args.graph),
+                       slots=1)
+    sids = [srv.open_stream(s["H"], s["W"], stream_cfg(core, args)) for s in streams]
     for t in range(0, args.warmup + 1):
         for sid, s in zip(sids, streams):
             srv.submit(sid, s["vid"][t], s["flows"][t - 1] if t > 0 else None)
@@ -144,36 +153,111 @@ def run_served(args, streams, torch, core):
         while srv.pending():
             srv.step()
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    lat = [r.done - r.arrival for r in srv.completed]
-    # dominant kernel: the batched fused sweep (all 16 frames per launch) of
-    # the last step's batch, launched back to back between two HIP events on
-    # the serving stream (no event between launches, so no dispatch bubbles)
-    launches = SWEEP_TIMING_LAUNCHES
-    ms = srv.runner.time_sweeps(launches, stream=srv.cuda_stream)
-    search = {}
-    for st in srv.streams.values():  # split-proposal searches per path, per frame size
-        key = f"{st.H}x{st.W}"
-        c = st.ctx.split_search_counts()
-        search[key] = {k: search.get(key, {}).get(k, 0) + v for k, v in c.items()}
-    px = sum(s["H"] * s["W"] for s in streams)
-    per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
+    out = {"elapsed": time.perf_counter() - t0, "lat": [r.done - r.arrival for r in srv.completed]}
+    if timing:
+        # dominant kernel: the batched fused sweep (all 16 frames per launch)
+        # of the last step's batch, launched back to back between two HIP
+        # events on the serving stream (no event between launches)
+        launches = SWEEP_TIMING_LAUNCHES
+        sweep_ms = srv.runner.time_sweeps(launches, stream=srv.cuda_stream)
+        search = {}
+        for st in srv.streams.values():  # split-proposal searches per path, per frame size
+            key = f"{st.H}x{st.W}"
+            c = st.ctx.split_search_counts()
+            search[key] = {k: search.get(key, {}).get(k, 0) + v for k, v in c.items()}
+        px = sum(s["H"] * s["W"] for s in streams)
+        per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
+        out.update(sweep_ms=sweep_ms, sweep_launches=launches, split_search=search,
+                   sweep_bytes=launches * BYTES_PER_PX_SWEEP * per_launch * px)
     srv.close()
-    return {"elapsed": elapsed, "lat": lat, "sweep_ms": ms,
-            "sweep_bytes": launches * BYTES_PER_PX_SWEEP * per_launch * px,
-            "sweep_launches": launches, "split_search": search}
+    return out
+
+
+def run_open(args, streams, torch, core):
+    """Open loop: every stream a Poisson process of args.fps frames/s; the
+    main thread submits each frame at its arrival time while the worker
+    threads run batches on args.slots executors."""
+    from st_spix_amd import serve
+    core.set_sweep_shape(args.sweep_shape)
+    F = min(args.open_frames, streams[0]["vid"].shape[0])
+    srv = serve.Server(hbm_budget_gb=64, max_batch=len(streams), use_graph=bool(args.graph),
+                       slots=args.slots)
+    sids = [srv.open_stream(s["H"], s["W"], stream_cfg(core, args)) for s in streams]
+    arrivals = serve.poisson_arrivals(len(streams), F, args.fps, seed=7)
+    srv.start()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ta, si, f in arrivals:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        s = streams[si]
+        srv.submit(sids[si], s["vid"][f], s["flows"][f - 1] if f > 0 else None, arrival=t0 + ta)
+    srv.drain()
+    t_end = max(r.done for r in srv.completed)
+    srv.stop()
+    reqs = list(srv.completed)
+    frame_of = {}
+    for si, sid in enumerate(sids):
+        for k, r in enumerate([r for r in reqs if r.sid == sid]):
+            frame_of[id(r)] = k
+    prop_lat = [r.done - r.arrival for r in reqs if frame_of[id(r)] > 0]
+    all_lat = [r.done - r.arrival for r in reqs]
+    batches = sum(s.batches for s in srv._slots)
+    srv.close()
+    return {"frames": len(reqs), "elapsed": t_end - t0, "fps": len(reqs) / (t_end - t0),
+            "offered_fps": len(streams) * args.fps, "p50": pct(prop_lat, 50),
+            "p99": pct(prop_lat, 99), "p99_all": pct(all_lat, 99), "batches": batches,
+            "frames_per_stream": F}
+
+
+def open_loop_child(args):
+    """The open loop in its own process: generates the streams, runs the
+    open loop and prints its result as one JSON line."""
+    import torch
+
+    from st_spix_amd import core, synth
+    streams = make_streams(args, torch, synth, args.open_frames)
+    r = run_open(args, streams, torch, core)
+    print(json.dumps(r), flush=True)
+
+
+def run_open_child(args):
+    """Runs open_loop_child in a child process; None if it failed."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--open-loop-child",
+           "--open-frames", str(args.open_frames), "--fps", str(args.fps), "--slots",
+           str(args.slots), "--iters", str(args.iters), "--sp-size", str(args.sp_size),
+           "--sweep-shape", str(args.sweep_shape), "--graph", str(args.graph),
+           "--streams-per-res", str(args.streams_per_res)]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, text=True)
+    if p.returncode != 0:
+        log(f"open loop child failed with status {p.returncode}")
+        return None
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(args):
-    """The oracle on the host cores: one full tick of the stream mix (every
-    stream's propagated frame, same iteration count), one stream per thread
-    (the oracle has no shared mutable state; ctypes releases the GIL).  Frame
-    0 of each stream is an untimed 20-iteration BASS."""
+    """The oracle on the host cores, on a bounded sample of the workload: the
+    first propagated frame of every stream (100 iterations, after an
+    untimed 20-iteration BASS frame 0), one stream per thread on all the
+    cores this process may use; and single-threaded, the first propagated
+    frame of one 360x640 and one 480x854 stream."""
     import concurrent.futures as cf
 
     import oracle
     from st_spix_amd import synth
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = max(1, len(os.sched_getaffinity(0)))
     jobs = []
     for ri, (H, W) in enumerate(RESOLUTIONS):
         for j in range(args.streams_per_res):
@@ -188,17 +272,28 @@ def cpu_baseline(args):
     def tick(i):
         vid, flows = jobs[i]
         seg, p = states[i]
-        return oracle.prop(vid[1], flows[0], seg, p, cfg)
+        return oracle.prop(vid[1], flows[0], seg, {k: v.copy() for k, v in p.items()}, cfg)
 
-    with cf.ThreadPoolExecutor(threads) as ex:
+    with cf.ThreadPoolExecutor(min(threads, len(jobs))) as ex:
         states = list(ex.map(first, jobs))
         t0 = time.perf_counter()
         list(ex.map(tick, range(len(jobs))))
         dt = time.perf_counter() - t0
-    return {"value": len(jobs) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C restatement of the reference), {threads} threads, one stream "
-                      f"per thread: one tick of all {len(jobs)} streams (a propagated frame each, "
-                      f"{args.iters} iterations) in {dt:.2f} s"}
+    single = [0, args.streams_per_res]  # the first 360x640 and 480x854 streams
+    t1 = time.perf_counter()
+    for i in single:
+        tick(i)
+    dt1 = time.perf_counter() - t1
+    return {"value": len(jobs) / dt, "unit": "frames/s", "cores": min(threads, len(jobs)),
+            "kind": "port",
+            "sample": f"oracle (C restatement of the reference) on {cpu_model()} "
+                      f"(nproc {os.cpu_count()}, {threads} usable): the first propagated frame "
+                      f"of all {len(jobs)} streams ({args.iters} iterations) on "
+                      f"{min(threads, len(jobs))} threads in {dt:.2f} s",
+            "single_thread": {"frames_per_s": round(len(single) / dt1, 4),
+                              "sample": "first propagated frame of one 360x640 and one 480x854 "
+                                        f"stream, 1 thread, {dt1:.2f} s"},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "usable_cores": threads}
 
 
 def sweep_traffic(kernel):
@@ -226,6 +321,15 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
+    if args.open_loop_child:
+        open_loop_child(args)
+        return
+    opened = None
+    if not args.no_open:  # before this process touches the GPU
+        opened = run_open_child(args)
+        if opened is not None:
+            log(f"open loop: {opened['frames']} frames in {opened['elapsed']:.3f} s, "
+                f"p50 {opened['p50'] * 1e3:.1f} ms p99 {opened['p99'] * 1e3:.1f} ms")
     import torch
 
     from st_spix_amd import core, synth
@@ -237,10 +341,10 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
         del src, dst
-    streams = make_streams(args, torch, synth)
+    streams = make_streams(args, torch, synth, args.warmup + args.steps + 1)
     log(f"{len(streams)} synthetic streams resident in HBM")
-    served = run_served(args, streams, torch, core)
-    log(f"served: {served['elapsed']:.3f} s for {args.steps} steps")
+    served = run_closed(args, streams, torch, core)
+    log(f"served closed loop: {served['elapsed']:.3f} s for {args.steps} steps")
     log(f"split searches per path: {served['split_search']}")
     plain = None if args.no_plain else run_plain(args, streams, torch, core)
     if plain is not None:
@@ -259,12 +363,17 @@ def main():
         "config": {"workload": "16 streams (4 each 360x640, 480x854, 720x1280, 1080x1920), "
                                f"propagated frames, {args.iters} iterations x 4 sweeps, "
                                f"sp_size {args.sp_size}, split/merge every 4",
-                   "mode": "served: one batch per step over all streams"
+                   "mode": "served closed loop: one batch per tick over all streams"
                            + (", refinement replayed from a hipGraph" if args.graph else ""),
+                   "latency": None if opened is None else
+                   f"open loop: {args.fps:g} frames/s per stream, Poisson arrivals, "
+                   f"{opened['frames_per_stream']} frames per stream, {args.slots} executors",
                    "sweep_shape": args.sweep_shape},
-        "p50_ms": round(pct(served["lat"], 50) * 1e3, 3),
-        "p99_ms": round(pct(served["lat"], 99) * 1e3, 3),
+        "p50_ms": ms(opened["p50"]) if opened else ms(pct(served["lat"], 50)),
+        "p99_ms": ms(opened["p99"]) if opened else ms(pct(served["lat"], 99)),
         "mpix_per_s": round(mpx, 2),
+        "closed_loop_latency": {"p50_ms": ms(pct(served["lat"], 50)),
+                                "p99_ms": ms(pct(served["lat"], 99))},
         "roofline": {"bound": "hbm",
                      "kernel": kernel + " (fused sweeps over all frames of the step)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -272,11 +381,15 @@ def main():
                      "launches": served["sweep_launches"],
                      "avg_launch_ms": round(served["sweep_ms"] / max(1, served["sweep_launches"]), 4)},
     }
+    if opened is not None:
+        line["open_loop"] = {"fps": round(opened["fps"], 3), "offered_fps": opened["offered_fps"],
+                             "p50_ms": ms(opened["p50"]), "p99_ms": ms(opened["p99"]),
+                             "p99_all_frames_ms": ms(opened["p99_all"]),
+                             "batches": opened["batches"]}
     if plain is not None:
         pfps = nframes / plain["elapsed"]
         line["plain"] = {"frames_per_s": round(pfps, 3),
-                         "p50_ms": round(pct(plain["lat"], 50) * 1e3, 3),
-                         "p99_ms": round(pct(plain["lat"], 99) * 1e3, 3)}
+                         "p50_ms": ms(pct(plain["lat"], 50)), "p99_ms": ms(pct(plain["lat"], 99))}
         line["served_over_plain"] = round(fps / pfps, 3)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

@@ -112,6 +112,11 @@ int stsp_ctx_set_split_search(stsp_ctx* ctx, int wave_box, int win_rows, int win
  * counterpart. */
 int stsp_ctx_split_search_counts(stsp_ctx* ctx, long long counts[4]);
 
+/* Device bytes the context holds now (its workspace grows with the label
+ * count of the frames it runs; a growth waits only for the context's own
+ * previous call, not for the device).  For admission bookkeeping. */
+int stsp_ctx_device_bytes(stsp_ctx* ctx, size_t* bytes);
+
 /* Capacity an output parameter set needs: BASS 30*K0 (bass.cu:245),
  * propagated frame 10*nspix_prev (prop.cu:702). */
 int stsp_bass_capacity(int sp_size, int height, int width);

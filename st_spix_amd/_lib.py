@@ -64,7 +64,7 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_prop_batch", "stsp_bass_batch",
            "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb", "stsp_ctx_set_split_search",
            "stsp_ctx_split_search_counts", "stsp_bass_noalloc", "stsp_bass_noalloc_scratch_bytes",
-           "stsp_prop_ex"]
+           "stsp_prop_ex", "stsp_ctx_device_bytes"]
 
 
 def lib() -> ctypes.CDLL:
@@ -87,6 +87,7 @@ def lib() -> ctypes.CDLL:
         L.stsp_bass.argtypes = [vp, vp, vp, ctypes.POINTER(ParamsStruct), pi, vp]
         L.stsp_prop.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ParamsStruct), i, vp,
                                 ctypes.POINTER(ParamsStruct), pi, pi, vp]
+        L.stsp_ctx_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.stsp_prop_ex.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ParamsStruct), i, vp,
                                    ctypes.POINTER(ParamsStruct), pi, pi, vp, vp, vp]
         L.stsp_marked_video.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float), i, i, i, vp, vp]

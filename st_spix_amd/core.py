@@ -87,6 +87,13 @@ class Context:
                    "stsp_ctx_split_search_counts")
         return dict(zip(("wave", "band", "window", "whole"), (int(v) for v in out)))
 
+    def device_bytes(self) -> int:
+        """Device memory the context holds (stsp_ctx_device_bytes)."""
+        n = ctypes.c_size_t(0)
+        _lib.check(_lib.lib().stsp_ctx_device_bytes(self.handle, ctypes.byref(n)),
+                   "stsp_ctx_device_bytes")
+        return int(n.value)
+
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(_lib.lib().stsp_ctx_enable_timing(self.handle, int(on)),
                    "stsp_ctx_enable_timing")

@@ -190,6 +190,11 @@ struct stsp_ctx {
   // workspace carved from the static scratch buffer: nothing is allocated,
   // the side streams are not used and the capacity is fixed
   bool noalloc = false;
+  // recorded on the caller's stream at the end of every call on this
+  // context: replacing a buffer waits for it rather than for the device
+  hipEvent_t done_ev = nullptr;
+  bool done_rec = false;
+  size_t pix_bytes = 0, lab_bytes = 0, prev_bytes = 0;
   int* host;  // pinned
   void* pix_mem = nullptr;
   void* lab_mem = nullptr;
@@ -233,7 +238,8 @@ static void carve_pixels(stsp_ctx* c, char* base) {
 }
 
 static int alloc_pixels(stsp_ctx* c) {
-  HIP_TRY(hipMalloc(&c->pix_mem, pixel_bytes(c->npix)));
+  c->pix_bytes = pixel_bytes(c->npix);
+  HIP_TRY(hipMalloc(&c->pix_mem, c->pix_bytes));
   carve_pixels(c, (char*)c->pix_mem);
   HIP_TRY(hipHostMalloc((void**)&c->host, 16 * sizeof(int), hipHostMallocDefault));
   return STSP_OK;
@@ -330,12 +336,14 @@ static int ensure_labels(stsp_ctx* c, int cap) {
   if (cap <= c->cap) return STSP_OK;
   if (c->noalloc) return STSP_ERR_TOO_LARGE;  // sized once, from the static scratch buffer
   if (c->lab_mem) {
-    HIP_TRY(hipDeviceSynchronize());
+    if (c->done_rec) HIP_TRY(hipEventSynchronize(c->done_ev));  // the last call's work
     HIP_TRY(hipFree(c->lab_mem));
     c->lab_mem = nullptr;
+    c->lab_bytes = 0;
   }
   cap = cap + cap / 4;  // headroom for growth across frames
   HIP_TRY(hipMalloc(&c->lab_mem, label_bytes(cap)));
+  c->lab_bytes = label_bytes(cap);
   const int rc = carve_labels(c, (char*)c->lab_mem, cap);
   invalidate_graph(&c->self);
   return rc;
@@ -344,13 +352,15 @@ static int ensure_labels(stsp_ctx* c, int cap) {
 static int ensure_prev(stsp_ctx* c, int n) {
   if (n <= c->nprev) return STSP_OK;
   if (c->prev_mem) {
-    HIP_TRY(hipDeviceSynchronize());
+    if (c->done_rec) HIP_TRY(hipEventSynchronize(c->done_ev));  // the last call's work
     HIP_TRY(hipFree(c->prev_mem));
     c->prev_mem = nullptr;
+    c->prev_bytes = 0;
   }
   n = n + n / 4;
   size_t bytes = (size_t)n * (16 + 12 + 8 + 16 + 8) + 16 * 256;
   HIP_TRY(hipMalloc(&c->prev_mem, bytes));
+  c->prev_bytes = bytes;
   Arena a;
   a.base = (char*)c->prev_mem;
   a.size = bytes;
@@ -533,6 +543,14 @@ static int read_int(stsp_ctx* c, const int* dev, hipStream_t s, int* out) {
   HIP_TRY(hipMemcpyAsync(c->host, dev, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   *out = c->host[0];
+  return STSP_OK;
+}
+
+// end of a call on context c: everything it queued is ordered before this
+static int mark_done(stsp_ctx* c, hipStream_t s) {
+  if (!c->done_ev) return STSP_OK;
+  HIP_TRY(hipEventRecord(c->done_ev, s));
+  c->done_rec = true;
   return STSP_OK;
 }
 
@@ -1164,6 +1182,7 @@ static int run_single(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_o
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   rc = epilogue_finish(c, spix_out, out, nspix_out, s);
+  if (!rc) rc = mark_done(c, s);
   if (rc) return rc;
   if (c->timing) {
     HIP_TRY(hipStreamSynchronize(s));
@@ -1321,6 +1340,8 @@ int stsp_ctx_create(const stsp_config* cfg, int height, int width, stsp_ctx** ou
   c->lut = make_lut();
   int rc = alloc_pixels(c);
   if (!rc) rc = batch_alloc(&c->self, 1);
+  if (!rc && hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming) != hipSuccess)
+    rc = STSP_ERR_HIP;
   if (!rc && hipMemset(c->search_count, 0, sizeof(unsigned long long) * kSearchPaths) != hipSuccess)
     rc = STSP_ERR_HIP;
   if (rc) {
@@ -1337,6 +1358,7 @@ void stsp_ctx_destroy(stsp_ctx* c) {
   for (int k = 0; k < STSP_KCLASS_COUNT; k++)
     for (int i = 0; i < 2 * c->timers[k].created; i++) (void)hipEventDestroy(c->timers[k].ev[i]);
   batch_free(&c->self);
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   if (c->pix_mem) (void)hipFree(c->pix_mem);
   if (c->lab_mem) (void)hipFree(c->lab_mem);
   if (c->prev_mem) (void)hipFree(c->prev_mem);
@@ -1363,6 +1385,12 @@ int stsp_ctx_set_split_search(stsp_ctx* c, int wave_box, int win_rows, int win_c
   c->wave_box = wave_box;
   c->win_h = win_rows;
   c->win_w = win_cols;
+  return STSP_OK;
+}
+
+int stsp_ctx_device_bytes(stsp_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return STSP_ERR_ARG;
+  *bytes = c->pix_bytes + c->lab_bytes + c->prev_bytes;
   return STSP_OK;
 }
 
@@ -1515,6 +1543,10 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s));  // the counts (and the refinement's timing events)
   for (int i = 0; i < n; i++) tflush(ctxs[i]);  // timed launches are recorded on group leads
+  for (int i = 0; i < n; i++) {
+    rc = mark_done(ctxs[i], s);
+    if (rc) return rc;
+  }
   for (int i = 0; i < n; i++) {
     nspix_out[i] = ctxs[i]->nspix_prev + ctxs[i]->host[1];
     if (nspix_out[i] > outs[i].cap) return STSP_ERR_CAPACITY;

@@ -142,9 +142,12 @@ def batch_images(n: int = 32, lo=(240, 320), hi=(720, 1280), nregions: int = 40,
 
 def voronoi_video_torch(T: int, H: int, W: int, nregions: int = 120, vmax: float = 4.0,
                         noise: float = 0.03, flow_noise: float = 0.5, seed: int = 0,
-                        device="cuda"):
+                        device="cuda", churn: float = 0.0):
     """Same motion model as voronoi_video, generated on the GPU with torch
-    (used by the benchmark for large frames).  Returns (vid, flows) on device."""
+    (used by the benchmark and the long-video tests for large frames), with
+    the same ``churn`` rule: regions leaving the frame, or chosen with
+    probability ``churn`` per frame, re-enter from a random border point
+    with a new colour and velocity.  Returns (vid, flows) on device."""
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
     seeds = torch.stack([torch.rand(nregions, generator=g, dtype=torch.float64) * W,
@@ -183,8 +186,25 @@ def voronoi_video_torch(T: int, H: int, W: int, nregions: int = 120, vmax: float
         flows[t, ..., 1] = (v * scale).float() + flow_noise * torch.randn(
             (H, W), generator=gd, device=device)
         seeds = seeds + vel
-        for c, lim in ((0, W), (1, H)):
-            out = (seeds[:, c] < 0) | (seeds[:, c] > lim - 1)
-            vel[out, c] *= -1
-            seeds[:, c] = seeds[:, c].clamp(0, lim - 1)
+        if churn > 0:
+            out = (seeds[:, 0] < -W * 0.05) | (seeds[:, 0] > W * 1.05) | \
+                  (seeds[:, 1] < -H * 0.05) | (seeds[:, 1] > H * 1.05)
+            redo = (out.cpu() | (torch.rand(nregions, generator=g, dtype=torch.float64) < churn))
+            n = int(redo.sum())
+            if n:
+                side = torch.randint(0, 4, (n,), generator=g)
+                pos = torch.rand(n, generator=g, dtype=torch.float64)
+                sx = torch.where(side == 0, 0.0, torch.where(side == 1, W - 1.0, pos * W))
+                sy = torch.where(side == 2, 0.0, torch.where(side == 3, H - 1.0, pos * H))
+                idx = torch.nonzero(redo).flatten().to(device)
+                seeds[idx] = torch.stack([sx, sy], -1).to(device)
+                cols[idx] = torch.from_numpy(random_colors(rng, n)).to(device)
+                ang = torch.rand(n, generator=g, dtype=torch.float64) * 2 * math.pi
+                spd = torch.rand(n, generator=g, dtype=torch.float64) * vmax * 0.8
+                vel[idx] = torch.stack([spd * torch.cos(ang), spd * torch.sin(ang)], -1).to(device)
+        else:
+            for c, lim in ((0, W), (1, H)):
+                out = (seeds[:, c] < 0) | (seeds[:, c] > lim - 1)
+                vel[out, c] *= -1
+                seeds[:, c] = seeds[:, c].clamp(0, lim - 1)
     return vid, flows

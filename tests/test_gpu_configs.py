@@ -1,0 +1,236 @@
+"""The configuration workloads at their sizes on the GPU.
+
+Labels and parameters are compared bit for bit with the oracle through the
+committed digests of tests/golden/config_goldens.json (made by
+tests/golden/make_config_goldens.py from the same seeded inputs).  A test
+that rebuilds different inputs (the input digest does not match) runs the
+oracle itself instead.  Config 3 also checks that every one of the 16
+served streams gets the labels of the plain per-frame loop, and config 4
+reports the FP32 means against an FP64 recount of the GPU's own labels."""
+import json
+import math
+import os
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import config_workloads as wl
+from st_spix_amd import core, synth
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config_goldens.json")))
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host_params(p):
+    return {f: p[f].cpu().numpy() for f in wl.FIELDS}
+
+
+def oracle_frames(vid, flows, cfg, nframes):
+    import oracle
+    ocfg = oracle.make_cfg(**cfg)
+    seg, p = oracle.bass(vid[0], ocfg)
+    frames = [(seg, p)]
+    for t in range(1, nframes):
+        seg, p, _ = oracle.prop(vid[t], flows[t - 1], seg, p, ocfg)
+        frames.append((seg, p))
+    return [{"labels": wl.digest(s), "params": wl.params_digest(q)} for s, q in frames]
+
+
+def expected_frames(gold, vid, flows, cfg, nframes):
+    """The golden digests of the first nframes frames, or the oracle's own if
+    the inputs differ from those the digests were made from."""
+    if wl.digest(vid[:nframes], flows[: nframes - 1]) == gold["inputs"]:
+        return gold["frames"]
+    print("inputs differ from the golden inputs: running the oracle", flush=True)
+    return oracle_frames(vid, flows, cfg, nframes)
+
+
+def test_config1_480x854_100_iterations():
+    """Config 1's video (480x854, 120 regions, |v| <= 4 px/frame, flow
+    noise 0.5 px), first 5 frames, 100 iterations each."""
+    c = wl.CONFIG1
+    vid, flows = wl.config1_inputs()
+    exp = expected_frames(GOLD["config1"], vid, flows, c["cfg"], c["T"])
+    cfg = core.make_config(**c["cfg"])
+    spix, params = core.bass_frame(cuda(vid[0]), cfg)
+    got = [(spix, params)]
+    for t in range(1, c["T"]):
+        spix, params, _ = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg)
+        got.append((spix, params))
+    torch.cuda.synchronize()
+    for t, ((s, p), e) in enumerate(zip(got, exp)):
+        assert wl.digest(s.cpu().numpy()) == e["labels"], f"frame {t} labels"
+        assert wl.params_digest(host_params(p)) == e["params"], f"frame {t} params"
+
+
+def test_config2_32_images_one_batch_graph_replayed():
+    """Config 2: 32 images of sizes 240..720 x 320..1280 refined in shared
+    launches, the refinement captured once as a hipGraph and replayed on a
+    non-default stream; a second replay follows other eager GPU work."""
+    imgs = wl.config2_inputs()
+    gold = GOLD["config2"]
+    cfg = core.make_config(**wl.CONFIG2["cfg"])
+    same = all(wl.digest(im) == g["inputs"] for im, g in zip(imgs, gold))
+    if not same:
+        import oracle
+        print("inputs differ from the golden inputs: running the oracle", flush=True)
+        gold = []
+        for im in imgs:
+            s, p = oracle.bass(im, oracle.make_cfg(**wl.CONFIG2["cfg"]))
+            gold.append({"labels": wl.digest(s), "params": wl.params_digest(p)})
+    ctxs = [core.Context(im.shape[0], im.shape[1], cfg) for im in imgs]
+    dimgs = [cuda(im) for im in imgs]
+    runner = core.BatchRunner(32, use_graph=True)
+    stream = torch.cuda.Stream()
+    for rep in range(2):
+        if rep == 1:  # eager work on the default stream between the replays
+            x = torch.randn(4096, 4096, device="cuda")
+            (x @ x).sum().item()
+            core.bass_frame(dimgs[5], core.make_config(niters=6, sp_size=20))
+        with torch.cuda.stream(stream):
+            outs = runner.bass(list(zip(ctxs, dimgs)), stream=stream)
+        stream.synchronize()
+        for i, ((s, p), g) in enumerate(zip(outs, gold)):
+            assert wl.digest(s.cpu().numpy()) == g["labels"], f"replay {rep} image {i} labels"
+            assert wl.params_digest(host_params(p)) == g["params"], f"replay {rep} image {i}"
+    runner.close()
+    for c in ctxs:
+        c.close()
+
+
+RES = [(360, 640), (480, 854), (720, 1280), (1080, 1920)]
+
+
+def test_config3_16_streams_served_equals_plain():
+    """Config 3's mix (4 streams each at 360x640, 480x854, 720x1280 and
+    1080x1920) for 20 frames: every served frame equals the plain per-frame
+    loop's, and two streams (a 360x640 and a 1080x1920 one, the latter
+    leaving after 8 frames) match the oracle on their first 4 frames."""
+    from st_spix_amd import serve
+    cfg = core.make_config(**wl.CONFIG3_CFG)
+    T = 20
+    streams = []
+    for ri, (H, W) in enumerate(RES):
+        for j in range(4):
+            sample = 0 if (ri, j) == (0, 0) else 1 if (ri, j) == (3, 0) else None
+            if sample is not None:
+                Ts = T if sample == 0 else 8
+                vid, flows = wl.config3_sample_inputs(sample, T=Ts)
+                streams.append({"vid": cuda(vid), "flows": cuda(flows), "T": Ts,
+                                "sample": sample, "host": (vid, flows)})
+            else:
+                vid, flows = synth.voronoi_video_torch(T, H, W, nregions=120, vmax=4.0,
+                                                       seed=1000 * ri + j)
+                streams.append({"vid": vid, "flows": flows, "T": T, "sample": None})
+    torch.cuda.synchronize()
+    # served
+    srv = serve.Server(hbm_budget_gb=64, max_batch=16, use_graph=True)
+    sids = [srv.open_stream(s["vid"].shape[1], s["vid"].shape[2], cfg) for s in streams]
+    served = [[] for _ in streams]
+    for t in range(T):
+        for i, (sid, s) in enumerate(zip(sids, streams)):
+            if t < s["T"]:
+                srv.submit(sid, s["vid"][t], s["flows"][t - 1] if t > 0 else None)
+        while srv.pending():
+            srv.step()
+        for i, (sid, s) in enumerate(zip(sids, streams)):
+            if t < s["T"]:
+                served[i].append((srv.labels(sid).clone(), srv.streams[sid].params))
+            if t == s["T"] - 1 and sid in srv.streams:
+                srv.close_stream(sid)  # a stream that ends frees its budget
+    srv.close()
+    # plain per-frame loop
+    for i, s in enumerate(streams):
+        spix, params = core.bass_frame(s["vid"][0], cfg)
+        assert torch.equal(spix, served[i][0][0]), f"stream {i} frame 0"
+        for t in range(1, s["T"]):
+            spix, params, _ = core.prop_frame(s["vid"][t], s["flows"][t - 1], spix, params, cfg)
+            assert torch.equal(spix, served[i][t][0]), f"stream {i} frame {t}"
+    # oracle sample
+    for i, s in enumerate(streams):
+        if s["sample"] is None:
+            continue
+        g = GOLD["config3_sample"][s["sample"]]
+        n = wl.CONFIG3_SAMPLE[s["sample"]]["check"]
+        vid, flows = s["host"]
+        exp = expected_frames(g, vid[:n], flows[: n - 1], wl.CONFIG3_CFG, n)
+        for t in range(n):
+            assert wl.digest(served[i][t][0].cpu().numpy()) == exp[t]["labels"], \
+                f"sample stream {s['sample']} frame {t}"
+
+
+def fp64_means(img, spix, n):
+    """FP64 recount of per-label colour means, positions and sample
+    covariances from the final labels."""
+    H, W = spix.shape
+    lab = spix.flatten().long()
+    cnt = torch.zeros(n, dtype=torch.float64, device=img.device).index_add_(
+        0, lab, torch.ones_like(lab, dtype=torch.float64))
+    col = torch.zeros(n, 3, dtype=torch.float64, device=img.device).index_add_(
+        0, lab, img.reshape(-1, 3).double())
+    yy, xx = torch.meshgrid(torch.arange(H, device=img.device, dtype=torch.float64),
+                            torch.arange(W, device=img.device, dtype=torch.float64),
+                            indexing="ij")
+    pos = torch.stack([xx.flatten(), yy.flatten()], -1)
+    ps = torch.zeros(n, 2, dtype=torch.float64, device=img.device).index_add_(0, lab, pos)
+    mom = torch.stack([pos[:, 0] ** 2, pos[:, 0] * pos[:, 1], pos[:, 1] ** 2], -1)
+    ms = torch.zeros(n, 3, dtype=torch.float64, device=img.device).index_add_(0, lab, mom)
+    mu = ps / cnt[:, None]
+    samp = ms / cnt[:, None] - torch.stack([mu[:, 0] ** 2, mu[:, 0] * mu[:, 1], mu[:, 1] ** 2], -1)
+    return cnt, col / cnt[:, None], mu, samp
+
+
+def test_config4_fp32_against_fp64_report():
+    """Config 4's motion (1080x1920, 400 regions entering and leaving, sp 15,
+    split/merge every 2 iterations), 100 iterations per frame (24 frames by
+    default, STSP_CONFIG4_FRAMES for the 120 of the config): per frame,
+    the FP32 colour means and the FP64 positions and sample covariances
+    against an FP64 recount from the labels.  Tolerance: 1e-5 relative for
+    the FP32 means (measured against max(|mean|, 1e-2)), 1e-9 for FP64."""
+    T = int(os.environ.get("STSP_CONFIG4_FRAMES", "24"))
+    vid, flows = synth.voronoi_video_torch(T, 1080, 1920, nregions=400, vmax=4.0, churn=0.05,
+                                           seed=4)
+    cfg = core.make_config(niters=100, sp_size=15, potts=0.5, sm_period=2, prop_niters=100)
+    rows = []
+    spix = params = None
+    t0 = time.time()
+    for t in range(T):
+        ta = time.time()
+        if t == 0:
+            spix, params = core.bass_frame(vid[0], cfg)
+        else:
+            spix, params, _ = core.prop_frame(vid[t], flows[t - 1], spix, params, cfg)
+        torch.cuda.synchronize()
+        print(f"config4 frame {t}: {time.time() - ta:.3f} s, {params['ids'].shape[0]} labels",
+              flush=True)
+        n = params["ids"].shape[0]
+        # propagated frames keep the previous ids, so ids of labels that died
+        # are absent; the largest id is the last exported slot
+        assert int(spix.max()) == n - 1 and int(spix.min()) >= 0
+        cnt, mean, mu, samp = fp64_means(vid[t], spix, n)
+        assert torch.equal(cnt.int(), params["counts"]), f"frame {t}: counts"
+        live = cnt > 0
+        err_app = ((params["mu_app"].double() - mean).abs() /
+                   mean.abs().clamp(min=1e-2))[live].max().item()
+        err_pos = ((params["mu_shape"] - mu).abs() / mu.abs().clamp(min=1.0))[live].max().item()
+        err_cov = ((params["sample_sigma_shape"] - samp).abs() /
+                   samp.abs().clamp(min=1.0))[live].max().item()
+        rows.append({"frame": t, "nspix": int(n), "live": int(live.sum()),
+                     "mu_app_rel": err_app, "mu_shape_rel": err_pos, "sample_cov_rel": err_cov})
+        assert err_app <= 1e-5, rows[-1]
+        assert err_pos <= 1e-9 and err_cov <= 1e-9, rows[-1]
+    torch.cuda.synchronize()
+    out = os.environ.get("STSP_REPORT_DIR")
+    if out:
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, "config4_fp_report.json"), "w") as f:
+            json.dump({"frames": rows, "seconds": time.time() - t0}, f, indent=1)
+    print("config4 worst mu_app rel err", max(r["mu_app_rel"] for r in rows),
+          "nspix", rows[0]["nspix"], "->", rows[-1]["nspix"])

@@ -191,3 +191,102 @@ def test_time_sweeps_leaves_results_and_next_frame_unchanged(orc):
     runner.close()
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_prop_batch_mixed_schedules(orc, use_graph):
+    """One batch holding frames with different schedules (split/merge every
+    2, 3 and 4 iterations; adaptive and fixed iteration counts) is refined
+    in groups with separate table slices and graphs; every frame equals the
+    oracle's, on three batches in a row with eager work between them."""
+    T = 4
+    scheds = [dict(sm_period=2, prop_niters=0), dict(sm_period=3, prop_niters=6),
+              dict(sm_period=4, prop_niters=0), dict(sm_period=4, prop_niters=9)]
+    vids = make_videos(T)
+    ctxs, state, refs = [], [], []
+    side = torch.cuda.Stream()
+    for (H, W, sp, _), (vid, flows), sc in zip(STREAMS, vids, scheds):
+        kw = dict(niters=10, sp_size=sp, potts=10.0, **sc)
+        cfg = core.make_config(**kw)
+        ocfg = orc_cfg(orc, **kw)
+        ctx = core.Context(H, W, cfg)
+        with torch.cuda.stream(side):
+            spix, params = core.bass_frame(cuda(vid[0]), cfg, ctx=ctx)
+        seg, p = orc.bass(vid[0], ocfg)
+        frames = [(seg, p)]
+        for t in range(1, T):
+            seg, p, _ = orc.prop(vid[t], flows[t - 1], seg, p, ocfg)
+            frames.append((seg, p))
+        refs.append(frames)
+        ctxs.append(ctx)
+        state.append((spix, params))
+    runner = core.BatchRunner(8, use_graph=use_graph)
+    got = [[None] * T for _ in STREAMS]
+    with torch.cuda.stream(side):
+        for t in range(1, T):
+            items = [(ctxs[i], cuda(vids[i][0][t]), cuda(vids[i][1][t - 1]), state[i][0],
+                      state[i][1]) for i in range(len(STREAMS))]
+            outs = runner.prop(items, stream=side)
+            side.synchronize()
+            torch.randn(1 << 16, device="cuda").sum().item()  # eager work between replays
+            for i, (spix, params, _) in enumerate(outs):
+                assert_labels(spix, refs[i][t][0], f"stream{i} frame{t}")
+                state[i] = (spix, params)
+                got[i][t] = params
+    # prior/invalid counts of a frame are updated in place by the next one
+    for i in range(len(STREAMS)):
+        for t in range(1, T):
+            assert_params(got[i][t], refs[i][t][1], f"stream{i} frame{t}")
+    runner.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_server_continuous_batching_matches_alone():
+    """Worker threads on two executors, frames arriving over time from a
+    feeder thread while batches run: every frame's labels equal those of the
+    stream served alone, synchronously."""
+    import threading
+    import time
+    from st_spix_amd import serve
+    T = 4
+    vids = make_videos(T)
+    cfgs = [core.make_config(niters=10, sp_size=sp, potts=10.0, prop_niters=0)
+            for (_, _, sp, _) in STREAMS]
+    dv = [(cuda(v), cuda(f)) for v, f in vids]
+    # alone, synchronous
+    alone = []
+    for i in range(len(STREAMS)):
+        srv = serve.Server(hbm_budget_gb=8, max_batch=4, slots=1)
+        sid = srv.open_stream(STREAMS[i][0], STREAMS[i][1], cfgs[i])
+        frames = []
+        for t in range(T):
+            srv.submit(sid, dv[i][0][t], dv[i][1][t - 1] if t > 0 else None)
+            while srv.pending():
+                srv.step()
+            frames.append(srv.labels(sid).cpu().numpy())
+        srv.close()
+        alone.append(frames)
+    # mixed, continuous
+    srv = serve.Server(hbm_budget_gb=8, max_batch=3, slots=2, use_graph=True)
+    sids = [srv.open_stream(H, W, c) for (H, W, _, _), c in zip(STREAMS, cfgs)]
+    reqs = {i: [] for i in range(len(STREAMS))}
+    srv.start()
+
+    def feed():
+        for t in range(T):
+            for i, sid in enumerate(sids):
+                reqs[i].append(srv.submit(sid, dv[i][0][t], dv[i][1][t - 1] if t > 0 else None))
+                time.sleep(0.002 * (i + 1))
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    srv.drain(timeout=120)
+    srv.stop()
+    for i in range(len(STREAMS)):
+        for t in range(T):
+            np.testing.assert_array_equal(reqs[i][t].spix.cpu().numpy(), alone[i][t],
+                                          err_msg=f"stream {i} frame {t}")
+    assert sum(s.batches for s in srv._slots) >= T  # batches ran as frames arrived
+    srv.close()
