@@ -406,6 +406,59 @@ __device__ __forceinline__ void joint_b(const float* imgC, int x, int y, const S
   }
 }
 
+// Statistics moves of one tile, aggregated per label in LDS before the
+// global atomics: a linear-probing table of kMoveSlots labels with the net
+// change of count, colour sums, colour squares (int64) and tile-relative
+// position moments (int32, widened exactly when flushed).
+constexpr int kMoveSlots = 64;
+struct MoveTable {
+  long long col[kMoveSlots][6];  // S r,g,b then Q r,g,b
+  int pos[kMoveSlots][5];        // sum dx, dy, dx^2, dx dy, dy^2
+  int cnt[kMoveSlots];
+  int key[kMoveSlots];           // label, or -1
+};
+
+// slot of `label` (inserted if absent), or -1 when the table is full
+__device__ __forceinline__ int move_slot(MoveTable& T, int label) {
+  int h = (int)(((unsigned)label * 2654435761u) >> 26);  // 6 bits
+#pragma unroll 1
+  for (int probe = 0; probe < kMoveSlots; probe++) {
+    const int prev = atomicCAS(&T.key[h], -1, label);
+    if (prev == -1 || prev == label) return h;
+    h = (h + 1) & (kMoveSlots - 1);
+  }
+  return -1;
+}
+
+// adds sign * (this pixel's statistics) to label's entry, or straight to
+// the global sums when the table is full
+__device__ __forceinline__ void move_add(MoveTable& T, const LabelStats& st, int label, int sign,
+                                         const int64_t col[6], int dx, int dy, int64_t gx,
+                                         int64_t gy) {
+  const int h = move_slot(T, label);
+  if (h >= 0) {
+    atomicAdd(&T.cnt[h], sign);
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+      atomicAdd((unsigned long long*)&T.col[h][c], (unsigned long long)(sign * col[c]));
+    atomicAdd(&T.pos[h][0], sign * dx);
+    atomicAdd(&T.pos[h][1], sign * dy);
+    atomicAdd(&T.pos[h][2], sign * dx * dx);
+    atomicAdd(&T.pos[h][3], sign * dx * dy);
+    atomicAdd(&T.pos[h][4], sign * dy * dy);
+    return;
+  }
+  atomicAdd(st.count + label, sign);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    atomic_add_i64(st.S + 3 * label + c, sign * col[c]);
+    atomic_add_i64(st.Q + 3 * label + c, sign * col[3 + c]);
+  }
+  const int64_t v[5] = {gx, gy, gx * gx, gx * gy, gy * gy};
+#pragma unroll
+  for (int c = 0; c < 5; c++) atomic_add_i64(st.X + 5 * label + c, sign * v[c]);
+}
+
 __device__ __forceinline__ int nbits(const int nb[8], int label) {
   int num = 0;
 #pragma unroll
@@ -428,8 +481,12 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   constexpr int LW = TW + 2 * R, LH = TH + 2 * R;
   constexpr int RB0 = 4 * S - 1;  // widest border ring (sweep 0)
   constexpr int LCAP = ((TW + 2 * RB0 + 1) / 2) * ((TH + 2 * RB0 + 1) / 2);
+  constexpr int LISTB = 4 * LCAP * 2;
+  constexpr int POOLB = LISTB > (int)sizeof(MoveTable) ? LISTB : (int)sizeof(MoveTable);
   __shared__ int lab[LH * LW];
-  __shared__ unsigned short list[4][LCAP];
+  // the phase lists, and after the last phase the table of statistics moves
+  __shared__ __align__(16) unsigned char pool[POOLB];
+  unsigned short (*list)[LCAP] = reinterpret_cast<unsigned short (*)[LCAP]>(pool);
   __shared__ int cnt[4];
   __shared__ uint32_t lutb[8];
   __shared__ uint32_t touched[(TW * TH + 31) / 32];  // tile cells whose label moved
@@ -582,8 +639,21 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   }
   // write the tile; a pixel that changed label moves its exact integer
   // statistics from the old label to the new one, so the label statistics
-  // stay equal to a full recount (integer sums are order-free)
+  // stay equal to a full recount (integer sums are order-free).  The moves
+  // of the tile are summed per label in LDS (over the phase lists, no
+  // longer needed) and each label's net change is added once.
+  MoveTable& T = *reinterpret_cast<MoveTable*>(pool);
+  for (int i = tid; i < kMoveSlots; i += NT) {
+    T.key[i] = -1;
+    T.cnt[i] = 0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) T.col[i][c] = 0;
+#pragma unroll
+    for (int c = 0; c < 5; c++) T.pos[i][c] = 0;
+  }
+  __syncthreads();
   LabelStats st = F.st;
+  const int X0 = x0 + R, Y0 = y0 + R;  // the tile's first pixel
   for (int i = tid; i < TW * TH; i += NT) {
     int lx = R + i % TW, ly = R + i / TW;
     int gx = x0 + lx, gy = y0 + ly;
@@ -594,25 +664,35 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     if (!((touched[i >> 5] >> (i & 31)) & 1u)) continue;  // never moved: nl == seg_in[g]
     const int ol = seg_in[g];
     if (nl == ol) continue;
-    const int64_t x = gx, y = gy;
     // (the colour squares move too: the merge reads them)
-    const int64_t v[8] = {fx_color(img[3 * g]), fx_color(img[3 * g + 1]), fx_color(img[3 * g + 2]),
-                          x, y, x * x, x * y, y * y};
-    atomicAdd(st.count + ol, -1);
-    atomicAdd(st.count + nl, 1);
+    int64_t col[6];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const int64_t q = fx_square(img[3 * g + c]);
-      atomic_add_i64(st.S + 3 * ol + c, -v[c]);
-      atomic_add_i64(st.S + 3 * nl + c, v[c]);
-      atomic_add_i64(st.Q + 3 * ol + c, -q);
-      atomic_add_i64(st.Q + 3 * nl + c, q);
+      col[c] = fx_color(img[3 * g + c]);
+      col[3 + c] = fx_square(img[3 * g + c]);
     }
+    move_add(T, st, ol, -1, col, gx - X0, gy - Y0, gx, gy);
+    move_add(T, st, nl, 1, col, gx - X0, gy - Y0, gx, gy);
+  }
+  __syncthreads();
+  for (int i = tid; i < kMoveSlots; i += NT) {
+    const int u = T.key[i];
+    if (u < 0) continue;
+    const int64_t n = T.cnt[i], sdx = T.pos[i][0], sdy = T.pos[i][1];
+    if (n) atomicAdd(st.count + u, (int)n);
 #pragma unroll
-    for (int c = 0; c < 5; c++) {
-      atomic_add_i64(st.X + 5 * ol + c, -v[3 + c]);
-      atomic_add_i64(st.X + 5 * nl + c, v[3 + c]);
+    for (int c = 0; c < 3; c++) {
+      if (T.col[i][c]) atomic_add_i64(st.S + 3 * u + c, T.col[i][c]);
+      if (T.col[i][3 + c]) atomic_add_i64(st.Q + 3 * u + c, T.col[i][3 + c]);
     }
+    const int64_t XX0 = X0, YY0 = Y0;
+    const int64_t v[5] = {n * XX0 + sdx, n * YY0 + sdy,
+                          n * XX0 * XX0 + 2 * XX0 * sdx + T.pos[i][2],
+                          n * XX0 * YY0 + XX0 * sdy + YY0 * sdx + T.pos[i][3],
+                          n * YY0 * YY0 + 2 * YY0 * sdy + T.pos[i][4]};
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+      if (v[c]) atomic_add_i64(st.X + 5 * u + c, v[c]);
   }
 }
 
