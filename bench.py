@@ -55,8 +55,9 @@ def parse():
     ap.add_argument("--iters", type=int, default=100, help="refinement iterations per frame")
     ap.add_argument("--streams-per-res", type=int, default=4)
     ap.add_argument("--sp-size", type=int, default=25)
-    ap.add_argument("--sweep-shape", type=int, default=4,
-                    help="fused sweep launch shape (index into core.SWEEP_SHAPES)")
+    ap.add_argument("--sweep-shape", type=int, default=-1,
+                    help="fused sweep launch shape (index into core.SWEEP_SHAPES); "
+                         "-1: picked by the autotuner (core.SweepTuner)")
     ap.add_argument("--open-frames", type=int, default=60,
                     help="frames per stream of the open-loop (Poisson arrival) run")
     ap.add_argument("--fps", type=float, default=30.0, help="open-loop arrival rate per stream")
@@ -108,15 +109,38 @@ def stream_cfg(core, args):
                             prop_niters=args.iters)
 
 
+def tune_plain(args, streams, torch, core, cfg):
+    """Shape of the plain loop's sweeps per resolution: the autotuner's
+    probe over one propagated frame of that resolution (a one-frame batch
+    in a scratch context; its result is discarded)."""
+    tuner = core.SweepTuner()
+    runner = core.BatchRunner(1, use_graph=False, tuner=tuner)
+    t = max(args.warmup, 1)
+    chosen = {}
+    for s in streams:
+        key = (s["H"], s["W"])
+        if key in chosen:
+            continue
+        ctx = core.Context(s["H"], s["W"], cfg)
+        runner.prop([(ctx, s["vid"][t], s["flows"][t - 1], s["spix"], s["params"])])
+        torch.cuda.synchronize()
+        chosen[key] = ctx.sweep_shape
+        core.get_context(s["H"], s["W"], cfg, "cuda").set_tuning(ctx.sweep_shape)
+        ctx.close()
+    runner.close()
+    return {f"{h}x{w}": v for (h, w), v in chosen.items()}
+
+
 def run_plain(args, streams, torch, core):
     cfg = stream_cfg(core, args)
-    core.set_sweep_shape(args.sweep_shape)
+    core.set_sweep_shape(args.sweep_shape if args.sweep_shape >= 0 else core.DEFAULT_SWEEP_SHAPE)
     for s in streams:
         s["spix"], s["params"] = core.bass_frame(s["vid"][0], cfg)
     for t in range(1, args.warmup + 1):
         for s in streams:
             s["spix"], s["params"], _ = core.prop_frame(s["vid"][t], s["flows"][t - 1], s["spix"],
                                                         s["params"], cfg)
+    shapes = tune_plain(args, streams, torch, core, cfg) if args.sweep_shape < 0 else None
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -127,16 +151,17 @@ def run_plain(args, streams, torch, core):
                                                         s["params"], cfg)
             lat.append(time.perf_counter() - a)
     torch.cuda.synchronize()
-    return {"elapsed": time.perf_counter() - t0, "lat": lat}
+    return {"elapsed": time.perf_counter() - t0, "lat": lat, "sweep_shapes": shapes}
 
 
 def run_closed(args, streams, torch, core, timing=True):
     """Closed loop: each tick submits one frame of every stream at once; the
     server drains them (one batch of all streams per tick)."""
     from st_spix_amd import serve
-    core.set_sweep_shape(args.sweep_shape)
+    if args.sweep_shape >= 0:
+        core.set_sweep_shape(args.sweep_shape)
     srv = serve.Server(hbm_budget_gb=64, max_batch=len(streams), use_graph=bool(args.graph),
-                       slots=1)
+                       slots=1, autotune=args.sweep_shape < 0)
     sids = [srv.open_stream(s["H"], s["W"], stream_cfg(core, args)) for s in streams]
     for t in range(0, args.warmup + 1):
         for sid, s in zip(sids, streams):
@@ -154,6 +179,7 @@ def run_closed(args, streams, torch, core, timing=True):
             srv.step()
     torch.cuda.synchronize()
     out = {"elapsed": time.perf_counter() - t0, "lat": [r.done - r.arrival for r in srv.completed]}
+    log(f"served closed loop: {args.steps} steps done")
     if timing:
         # dominant kernel: the batched fused sweep (all 16 frames per launch)
         # of the last step's batch, launched back to back between two HIP
@@ -166,9 +192,12 @@ def run_closed(args, streams, torch, core, timing=True):
             c = st.ctx.split_search_counts()
             search[key] = {k: search.get(key, {}).get(k, 0) + v for k, v in c.items()}
         px = sum(s["H"] * s["W"] for s in streams)
-        per_launch = core.SWEEP_SHAPES[args.sweep_shape][2]
+        shape = next(iter(srv.streams.values())).ctx.sweep_shape  # the last batch's shape
+        per_launch = core.SWEEP_SHAPES[shape][2]
         out.update(sweep_ms=sweep_ms, sweep_launches=launches, split_search=search,
-                   sweep_bytes=launches * BYTES_PER_PX_SWEEP * per_launch * px)
+                   sweep_bytes=launches * BYTES_PER_PX_SWEEP * per_launch * px,
+                   sweep_shape=shape,
+                   autotune=srv.tuner.report() if srv.tuner is not None else None)
     srv.close()
     return out
 
@@ -178,10 +207,11 @@ def run_open(args, streams, torch, core):
     main thread submits each frame at its arrival time while the worker
     threads run batches on args.slots executors."""
     from st_spix_amd import serve
-    core.set_sweep_shape(args.sweep_shape)
+    if args.sweep_shape >= 0:
+        core.set_sweep_shape(args.sweep_shape)
     F = min(args.open_frames, streams[0]["vid"].shape[0])
     srv = serve.Server(hbm_budget_gb=64, max_batch=len(streams), use_graph=bool(args.graph),
-                       slots=args.slots)
+                       slots=args.slots, autotune=args.sweep_shape < 0)
     sids = [srv.open_stream(s["H"], s["W"], stream_cfg(core, args)) for s in streams]
     arrivals = serve.poisson_arrivals(len(streams), F, args.fps, seed=7)
     srv.start()
@@ -353,7 +383,7 @@ def main():
     fps = nframes / served["elapsed"]
     mpx = sum(s["H"] * s["W"] for s in streams) * args.steps / served["elapsed"] / 1e6
     achieved = served["sweep_bytes"] / (served["sweep_ms"] / 1e3) / 1e9 if served["sweep_ms"] else 0
-    kernel = "kb_sweep<%d,%d,%d>" % tuple(core.SWEEP_SHAPES[args.sweep_shape])
+    kernel = "kb_sweep<%d,%d,%d>" % tuple(core.SWEEP_SHAPES[served["sweep_shape"]])
     line = {
         "metric": METRIC, "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -368,7 +398,8 @@ def main():
                    "latency": None if opened is None else
                    f"open loop: {args.fps:g} frames/s per stream, Poisson arrivals, "
                    f"{opened['frames_per_stream']} frames per stream, {args.slots} executors",
-                   "sweep_shape": args.sweep_shape},
+                   "sweep_shape": served["sweep_shape"],
+                   "sweep_shape_from": "autotuner" if args.sweep_shape < 0 else "--sweep-shape"},
         "p50_ms": ms(opened["p50"]) if opened else ms(pct(served["lat"], 50)),
         "p99_ms": ms(opened["p99"]) if opened else ms(pct(served["lat"], 99)),
         "mpix_per_s": round(mpx, 2),
@@ -386,10 +417,13 @@ def main():
                              "p50_ms": ms(opened["p50"]), "p99_ms": ms(opened["p99"]),
                              "p99_all_frames_ms": ms(opened["p99_all"]),
                              "batches": opened["batches"]}
+    if served.get("autotune") is not None:
+        line["autotune"] = served["autotune"]
     if plain is not None:
         pfps = nframes / plain["elapsed"]
         line["plain"] = {"frames_per_s": round(pfps, 3),
-                         "p50_ms": ms(pct(plain["lat"], 50)), "p99_ms": ms(pct(plain["lat"], 99))}
+                         "p50_ms": ms(pct(plain["lat"], 50)), "p99_ms": ms(pct(plain["lat"], 99)),
+                         "sweep_shapes": plain["sweep_shapes"]}
         line["served_over_plain"] = round(fps / pfps, 3)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

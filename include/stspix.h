@@ -224,6 +224,20 @@ int stsp_ctx_kernel_time(stsp_ctx* ctx, int kclass, double* total_ms, long long*
  * benchmarks (no counterpart in the reference). */
 int stsp_batch_time_sweeps(stsp_batch* batch, int nlaunch, hipStream_t stream, float* ms_out);
 
+/* Sweep-shape autotuner probe (no counterpart in the reference, whose sweep
+ * launch shape is fixed, update_seg.cu:281-311).  Over the frames of the
+ * batch's last refinement, and like stsp_batch_time_sweeps leaving returned
+ * results unchanged, times `nlaunch` launches of every fused-sweep launch
+ * shape (tile width x height x sweeps per launch, STSP_SWEEP_SHAPES of
+ * them, indices as stsp_ctx_set_tuning) after one untimed launch each;
+ * us_per_sweep[STSP_SWEEP_SHAPES] receives microseconds per sweep and *best
+ * the shape with the cheapest iteration (the sweeps per iteration of the
+ * last refinement, as the refinement splits them into launches).  Blocks
+ * until the probe is done.  All shapes give bit-identical labels. */
+enum { STSP_SWEEP_SHAPES = 6 };
+int stsp_batch_tune_sweeps(stsp_batch* batch, int nlaunch, hipStream_t stream,
+                           float* us_per_sweep, int* best);
+
 const char* stsp_status_string(int status);
 
 #ifdef __cplusplus

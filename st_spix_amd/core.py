@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from typing import Dict, Optional, Tuple
 
 import torch
@@ -67,10 +68,12 @@ class Context:
             _lib.check(_lib.lib().stsp_ctx_create(ctypes.byref(cfg), self.H, self.W,
                                                   ctypes.byref(h)), "stsp_ctx_create")
         self.handle = h
+        self.sweep_shape = DEFAULT_SWEEP_SHAPE
 
     def set_tuning(self, sweep_shape: int) -> None:
         _lib.check(_lib.lib().stsp_ctx_set_tuning(self.handle, int(sweep_shape)),
                    "stsp_ctx_set_tuning")
+        self.sweep_shape = int(sweep_shape)
 
     def set_split_search(self, wave_box: int = 128, win_rows: int = 448,
                          win_cols: int = 384) -> None:
@@ -281,17 +284,99 @@ def marked_video(vid: torch.Tensor, spix: torch.Tensor, color) -> torch.Tensor:
     return marked
 
 
-class BatchRunner:
-    """Refines several propagated frames (one per Context) in shared launches
-    (stsp_prop_batch), optionally replaying a captured hipGraph."""
+class SweepTuner:
+    """Autotuner of the fused sweep's launch shape (tile x sweeps per launch).
 
-    def __init__(self, max_frames: int = 16, use_graph: bool = False):
+    Batches are keyed by their composition: the kind of frame (BASS first
+    frames or propagated frames), the total pixel count and the largest
+    frame, each rounded to half an octave.  The first batch of a key runs
+    with the current shape; after its results are out, one probe
+    (stsp_batch_tune_sweeps) times every shape on that batch's frames and
+    the cheapest becomes the key's shape for every later batch.  Shapes give
+    bit-identical results, so the choice changes speed only.  One tuner may
+    be shared by several runners."""
+
+    def __init__(self, nlaunch: int = 4):
+        self.nlaunch = int(nlaunch)
+        self.chosen = {}   # key -> shape index
+        self.probes = {}   # key -> microseconds per sweep of every shape
+        self._lock = threading.Lock()
+
+    @staticmethod
+    def key(kind: str, ctxs) -> tuple:
+        px = [c.H * c.W for c in ctxs]
+        return (kind, int(round(2 * math.log2(sum(px)))), int(round(2 * math.log2(max(px)))))
+
+    def shape_for(self, key):
+        with self._lock:
+            return self.chosen.get(key)
+
+    def probe(self, runner, key, stream) -> int:
+        us = (ctypes.c_float * _lib.STSP_SWEEP_SHAPES)()
+        best = ctypes.c_int()
+        _lib.check(_lib.lib().stsp_batch_tune_sweeps(runner.handle, self.nlaunch,
+                                                     ctypes.c_void_p(stream.cuda_stream), us,
+                                                     ctypes.byref(best)),
+                   "stsp_batch_tune_sweeps")
+        with self._lock:
+            self.chosen[key] = int(best.value)
+            self.probes[key] = [round(float(x), 3) for x in us]
+        return int(best.value)
+
+    def report(self) -> list:
+        with self._lock:
+            return [{"kind": k[0], "total_px_log2x2": k[1], "max_px_log2x2": k[2],
+                     "shape": self.chosen[k], "sweep": list(SWEEP_SHAPES[self.chosen[k]]),
+                     "us_per_sweep": self.probes.get(k)} for k in sorted(self.chosen)]
+
+
+class BatchRunner:
+    """Refines several frames (one per Context) in shared launches
+    (stsp_prop_batch, stsp_bass_batch), optionally replaying a captured
+    hipGraph and picking the sweep launch shape with a SweepTuner."""
+
+    def __init__(self, max_frames: int = 16, use_graph: bool = False,
+                 tuner: Optional["SweepTuner"] = None, defer_probe: bool = False):
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().stsp_batch_create(int(max_frames), ctypes.byref(h)),
                    "stsp_batch_create")
         self.handle = h
         self.max_frames = int(max_frames)
+        self.tuner = tuner
+        # with defer_probe the caller runs a batch's pending probe itself
+        # (probe_pending), e.g. after recording the batch's completion
+        self.defer_probe = bool(defer_probe)
+        self._pending = None
         _lib.check(_lib.lib().stsp_batch_use_graph(h, int(use_graph)), "stsp_batch_use_graph")
+
+    def _tuned(self, kind, ctxs):
+        """The tuner's key for this batch; applies its shape if it has one."""
+        if self.tuner is None:
+            return None
+        key = SweepTuner.key(kind, ctxs)
+        shape = self.tuner.shape_for(key)
+        if shape is not None:
+            for c in ctxs:
+                c.set_tuning(shape)
+            return None
+        return key
+
+    def _probe(self, key, ctxs, s):
+        if key is None:
+            return
+        self._pending = (key, ctxs, s)
+        if not self.defer_probe:
+            self.probe_pending()
+
+    def probe_pending(self) -> None:
+        """Runs the tuner's probe over the last batch if it asked for one."""
+        if self._pending is None:
+            return
+        key, ctxs, s = self._pending
+        self._pending = None
+        shape = self.tuner.probe(self, key, s)
+        for c in ctxs:
+            c.set_tuning(shape)
 
     def close(self) -> None:
         if self.handle:
@@ -345,9 +430,11 @@ class BatchRunner:
             outs[i] = params_struct(out, cap)
             keep.append((spix, out))
         s = stream if stream is not None else torch.cuda.current_stream()
+        key = self._tuned("bass", [it[0] for it in items])
         rc = _lib.lib().stsp_bass_batch(self.handle, ctxs, n, imgs, souts, outs, nout,
                                         ctypes.c_void_p(s.cuda_stream))
         _lib.check(rc, "stsp_bass_batch")
+        self._probe(key, [it[0] for it in items], s)
         return [(keep[i][0], trim(keep[i][1], nout[i])) for i in range(n)]
 
     def prop(self, items, stream=None):
@@ -386,7 +473,9 @@ class BatchRunner:
             nprev[i] = np_
             keep.append((spix, out))
         s = stream if stream is not None else torch.cuda.current_stream()
+        key = self._tuned("prop", [it[0] for it in items])
         rc = _lib.lib().stsp_prop_batch(self.handle, ctxs, n, imgs, flows, sprev, prevs, nprev,
                                         souts, outs, nout, niters, ctypes.c_void_p(s.cuda_stream))
         _lib.check(rc, "stsp_prop_batch")
+        self._probe(key, [it[0] for it in items], s)
         return [(keep[i][0], trim(keep[i][1], nout[i]), niters[i]) for i in range(n)]

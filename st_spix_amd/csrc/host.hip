@@ -144,6 +144,7 @@ struct stsp_batch {
   // the last refinement's launch settings (for stsp_batch_time_sweeps)
   SimpleLut last_lut = {};
   int last_shape = -1;
+  int last_nseg = 0;  // sweeps per iteration of the last refinement
   // propagation prologue: per-frame table, block offsets, per-round hole
   // counts ([kFillMax][nf] on the device, one chunk of rounds on the host)
   ProFrame* d_pro = nullptr;
@@ -840,6 +841,7 @@ static int refine(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s, int bas
   if (rc) return rc;
   bh->last_lut = lead->lut;
   bh->last_shape = shape;
+  bh->last_nseg = lead->cfg.niters_seg;
   int cur;
   bool timing = lead->timing != 0;
   // The legacy default stream cannot be captured: there the launches run
@@ -1489,6 +1491,50 @@ int stsp_batch_time_sweeps(stsp_batch* bh, int nlaunch, hipStream_t s, float* ms
   (void)hipEventDestroy(e[1]);
   HIP_TRY(hipGetLastError());
   return er == hipSuccess ? STSP_OK : STSP_ERR_HIP;
+}
+
+int stsp_batch_tune_sweeps(stsp_batch* bh, int nlaunch, hipStream_t s, float* us_per_sweep,
+                           int* best) {
+  if (!bh || !us_per_sweep || !best || nlaunch < 1 || bh->last_shape < 0) return STSP_ERR_ARG;
+  hipEvent_t e[2];
+  HIP_TRY(hipEventCreateWithFlags(&e[0], hipEventDisableSystemFence));
+  HIP_TRY(hipEventCreateWithFlags(&e[1], hipEventDisableSystemFence));
+  float launch_us[kSweepShapes];
+  hipError_t er = hipSuccess;
+  for (int sh = 0; sh < kSweepShapes && er == hipSuccess; sh++) {
+    // every shape starts from the same recount of the final labels
+    b_update_params(bh->b, 0, false, s);
+    const int per = kSweepShape[sh].s;
+    (void)b_sweeps(bh->b, 0, per, sh, bh->last_lut, s);  // untimed first launch
+    er = hipEventRecord(e[0], s);
+    for (int i = 0; i < nlaunch && er == hipSuccess; i++)
+      (void)b_sweeps(bh->b, 0, per, sh, bh->last_lut, s);
+    if (er == hipSuccess) er = hipEventRecord(e[1], s);
+    if (er == hipSuccess) er = hipEventSynchronize(e[1]);
+    float ms = 0.f;
+    if (er == hipSuccess) er = hipEventElapsedTime(&ms, e[0], e[1]);
+    launch_us[sh] = ms * 1000.f / nlaunch;
+    us_per_sweep[sh] = launch_us[sh] / per;
+  }
+  (void)hipEventDestroy(e[0]);
+  (void)hipEventDestroy(e[1]);
+  HIP_TRY(er);
+  HIP_TRY(hipGetLastError());
+  // the cost of one iteration's sweeps as b_sweeps runs them: launches of
+  // the shape, the remainder with the one-sweep shape
+  const int nseg = bh->last_nseg > 0 ? bh->last_nseg : 1;
+  int arg = 0;
+  float cost_best = 0.f;
+  for (int sh = 0; sh < kSweepShapes; sh++) {
+    const int per = kSweepShape[sh].s;
+    const float cost = (nseg / per) * launch_us[sh] + (nseg % per) * launch_us[0];
+    if (sh == 0 || cost < cost_best) {
+      cost_best = cost;
+      arg = sh;
+    }
+  }
+  *best = arg;
+  return STSP_OK;
 }
 
 int stsp_batch_use_graph(stsp_batch* bh, int on) {

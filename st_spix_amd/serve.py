@@ -13,7 +13,9 @@ Whenever an executor is free it takes the oldest pending frame of every
 stream that has one and is not already on the GPU (at most ``max_batch``),
 runs BASS for streams without state and the batched propagation +
 refinement (stsp_prop_batch) for the rest, and records each frame's latency
-from its arrival to its completion.  Frames that arrive while batches run
+from its arrival to its completion.  With ``autotune`` the executors share a
+sweep-shape tuner (core.SweepTuner), which probes the launch shapes once per
+batch composition.  Frames that arrive while batches run
 are admitted into the next free executor's batch, so the work of
 consecutive batches overlaps on the device.  The library call releases the
 Python interpreter lock, so executors wait on the GPU concurrently.
@@ -105,15 +107,16 @@ class Stream:
 
 
 class _Slot:
-    def __init__(self, max_batch: int, use_graph: bool, device):
-        self.runner = core.BatchRunner(max_batch, use_graph)
+    def __init__(self, max_batch: int, use_graph: bool, device, tuner):
+        self.runner = core.BatchRunner(max_batch, use_graph, tuner=tuner, defer_probe=True)
         self.stream = torch.cuda.Stream(device=device)  # default priority
         self.batches = 0
 
 
 class Server:
     def __init__(self, hbm_budget_gb: float = 64.0, max_batch: int = 16,
-                 use_graph: bool = False, device="cuda", slots: int = 2):
+                 use_graph: bool = False, device="cuda", slots: int = 2,
+                 autotune: bool = False):
         self.device = torch.device(device)
         self.admission = Admission(int(hbm_budget_gb * (1 << 30)))
         self.max_batch = max_batch
@@ -121,7 +124,10 @@ class Server:
         self.next_sid = 0
         self.completed: List[Request] = []
         self._cv = threading.Condition()
-        self._slots = [_Slot(max_batch, use_graph, self.device) for _ in range(max(1, slots))]
+        # one sweep-shape tuner shared by the executors (core.SweepTuner)
+        self.tuner = core.SweepTuner() if autotune else None
+        self._slots = [_Slot(max_batch, use_graph, self.device, self.tuner)
+                       for _ in range(max(1, slots))]
         self._workers: List[threading.Thread] = []
         self._running = False
         self._error: Optional[BaseException] = None
@@ -225,6 +231,8 @@ class Server:
         t = time.perf_counter()
         for _, r in batch:
             r.done = t
+        with torch.cuda.device(self.device):
+            slot.runner.probe_pending()  # once per batch composition, after the frames are out
         for st, _ in batch:
             held = st.ctx.device_bytes()
             if held > st.nbytes:

@@ -290,3 +290,44 @@ def test_server_continuous_batching_matches_alone():
                                           err_msg=f"stream {i} frame {t}")
     assert sum(s.batches for s in srv._slots) >= T  # batches ran as frames arrived
     srv.close()
+
+
+def test_sweep_tuner_picks_shape_and_keeps_results(orc):
+    """The sweep-shape autotuner (core.SweepTuner): the first batch of a
+    composition runs with the default shape and is probed (six shapes timed,
+    one chosen); later batches run with the chosen shape, in a new graph.
+    Every frame, before and after the switch, equals the oracle's."""
+    T = 4
+    kw = dict(niters=12, potts=10.0, prop_niters=10)
+    vids = make_videos(T)
+    ref = run_oracle(orc, vids, kw)
+    tuner = core.SweepTuner(nlaunch=3)
+    runner = core.BatchRunner(8, use_graph=True, tuner=tuner)
+    side = torch.cuda.Stream()
+    ctxs = [core.Context(H, W, core.make_config(sp_size=sp, **kw)) for H, W, sp, _ in STREAMS]
+    with torch.cuda.stream(side):
+        outs = runner.bass([(c, cuda(v[0][0])) for c, v in zip(ctxs, vids)], stream=side)
+        side.synchronize()
+        state = list(outs)
+        for i, (spix, params) in enumerate(outs):
+            assert_labels(spix, ref[i][0][0], f"stream{i} frame0")
+            assert_params(params, ref[i][0][1], f"stream{i} frame0")
+        for t in range(1, T):
+            items = [(ctxs[i], cuda(vids[i][0][t]), cuda(vids[i][1][t - 1]), state[i][0],
+                      state[i][1]) for i in range(len(STREAMS))]
+            outs = runner.prop(items, stream=side)
+            side.synchronize()
+            for i, (spix, params, _) in enumerate(outs):
+                assert_labels(spix, ref[i][t][0], f"stream{i} frame{t}")
+                assert_params(params, ref[i][t][1], f"stream{i} frame{t}")
+                state[i] = (spix, params)
+    rep = tuner.report()
+    assert sorted(r["kind"] for r in rep) == ["bass", "prop"], rep
+    for r in rep:
+        assert len(r["us_per_sweep"]) == 6 and all(u > 0 for u in r["us_per_sweep"]), r
+        assert 0 <= r["shape"] < 6
+    prop_shape = [r["shape"] for r in rep if r["kind"] == "prop"][0]
+    assert all(c.sweep_shape == prop_shape for c in ctxs)
+    runner.close()
+    for c in ctxs:
+        c.close()
