@@ -531,27 +531,29 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
     if (tid < 4) cnt[tid] = 0;
     __syncthreads();
     {
+      // The region starts one cell inside the LDS tile (rb < R), so the four
+      // neighbour reads of every cell are in the tile; they are issued
+      // together (no short-circuit), which is faster than testing the frame
+      // edges first.
       const int rb = 4 * (S - 1 - s) + 3;
+      static_assert(4 * (S - 1) + 3 < R, "the flag region keeps one cell of margin");
       const int bx0 = R - rb, by0 = R - rb, bw = TW + 2 * rb, bh = TH + 2 * rb;
       for (int i = tid; i < bw * bh; i += NT) {
         const int lx = bx0 + i % bw, ly = by0 + i / bw;
         const int gx = x0 + lx, gy = y0 + ly;
-        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
         const int li = ly * LW + lx;
-        bool bb;
-        if (gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1) {
-          bb = true;
-        } else {
-          const int C = lab[li];
-          bb = (C != lab[li - LW]) || (C != lab[li + LW]) || (C != lab[li + 1]) ||
-               (C != lab[li - 1]);
-        }
-        if (!bb) continue;
+        const int C = lab[li];
+        const bool diff = (C != lab[li - LW]) | (C != lab[li + LW]) | (C != lab[li + 1]) |
+                          (C != lab[li - 1]);
+        const bool edge = gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1;
+        const bool inside = gx >= 0 && gx < W && gy >= 0 && gy < H;
         const int p = ((gx & 1) << 1) | (gy & 1);
         const int rp = 4 * (S - 1 - s) + (3 - p);
-        if (lx < R - rp || lx >= R + TW + rp || ly < R - rp || ly >= R + TH + rp) continue;
-        const int pos = atomicAdd(&cnt[p], 1);
-        list[p][pos] = (unsigned short)li;
+        const bool inreg = lx >= R - rp && lx < R + TW + rp && ly >= R - rp && ly < R + TH + rp;
+        if (inside && (edge || diff) && inreg) {
+          const int pos = atomicAdd(&cnt[p], 1);
+          list[p][pos] = (unsigned short)li;
+        }
       }
       __syncthreads();
     }
@@ -592,16 +594,17 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
         const int cntC = rec[C].count;
         const int gidx = gy * W + gx;
         const float imgC[3] = {img[3 * gidx], img[3 * gidx + 1], img[3 * gidx + 2]};
+        // the four table words are read together (no short-circuit)
         int m[4];
-        bool ok = true;
+        uint32_t w[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          m[j] = 0;
-          if (!dup[j]) {
-            m[j] = nbits(nb, q[j]);
-            ok = ok && ((lutb[m[j] >> 5] >> (m[j] & 31)) & 1u);
-          }
+          m[j] = nbits(nb, q[j]);
+          w[j] = lutb[m[j] >> 5];
         }
+        uint32_t ok = 1u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) ok &= (uint32_t)dup[j] | ((w[j] >> (m[j] & 31)) & 1u);
         if (!ok) continue;
         float best = -9999.0f;
         int out = C;
@@ -1086,15 +1089,42 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
       }
     }
   }
+  // Cell (c, w) is word w of chunk c over the wave's 64 lanes.  A cell can
+  // grow at a level only if it, or a horizontally adjacent cell of the same
+  // chunk, changed at the level before (vertical neighbours are lanes of the
+  // same cell, or halo rows replaced at a round's end, which marks the cell
+  // too).  `act` (wave-uniform, bit c * WD + w) holds the cells that may
+  // grow; skipped cells would compute N = 0, so results are unchanged.
+  static_assert(RB * WD <= 32, "one activity bit per cell");
+  constexpr uint32_t kRowBits = (1u << WD) - 1u;
+  auto spread = [&](uint32_t ch) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int c = 0; c < RB; c++) {
+      const uint32_t r = (ch >> (c * WD)) & kRowBits;
+      out |= ((r | (r << 1) | (r >> 1)) & kRowBits) << (c * WD);
+    }
+    return out;
+  };
+  uint32_t act = 0;
+#pragma unroll
+  for (int c = 0; c < RB; c++)
+#pragma unroll
+    for (int w = 0; w < WD; w++)
+      if (__ballot((V1[c][w] | V2[c][w]) != 0) != 0) act |= 1u << (c * WD + w);
+  act = spread(act);
   for (int round = 0;; round++) {
     uint32_t grew = 0;
 #pragma unroll 1
     for (int j = 0; j < kHalo; j++) {
+      uint32_t ch = 0;
 #pragma unroll
       for (int c = 0; c < RB; c++) {
         uint64_t N1[WD], N2[WD];
 #pragma unroll
         for (int w = 0; w < WD; w++) {
+          N1[w] = N2[w] = 0;
+          if (!((act >> (c * WD + w)) & 1u)) continue;  // wave-uniform
           uint64_t h1 = (V1[c][w] << 1) | (V1[c][w] >> 1), h2 = (V2[c][w] << 1) | (V2[c][w] >> 1);
           if (w > 0) {
             h1 |= V1[c][w - 1] >> 63;
@@ -1110,12 +1140,16 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
         }
 #pragma unroll
         for (int w = 0; w < WD; w++) {
+          if (!((act >> (c * WD + w)) & 1u)) continue;  // N1 = N2 = 0 there
           V1[c][w] |= N1[w];
           NK[c][w] |= N2[w] & V1[c][w];
           V2[c][w] |= N2[w];
-          if (__ballot(own && (N1[w] | N2[w]) != 0) != 0) grew = 1;
+          const uint64_t any = N1[w] | N2[w];
+          if (__ballot(any != 0) != 0) ch |= 1u << (c * WD + w);
+          if (__ballot(own && any != 0) != 0) grew = 1;
         }
       }
+      act = spread(ch);
     }
     // publish the outer own rows: lanes 4..7 (top) and 56..59 (bottom)
     const int wb = round & 1;
@@ -1139,6 +1173,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     // lanes 0..3 take the band above's bottom rows, lanes 60..63 the band
     // below's top rows (none above band 0 or below the last band: zeros)
     const bool up = lane < kHalo, dn = lane >= 64 - kHalo;
+    uint32_t ch = 0;
 #pragma unroll
     for (int c = 0; c < RB; c++) {
       const int b = NW * c + wv;
@@ -1149,10 +1184,15 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
       for (int w = 0; w < WD; w++) {
         const uint64_t e1 = ok ? E.v[wb][ok ? nb : 0][up ? 1 : 0][ok ? r : 0][0][w] : 0ull;
         const uint64_t e2 = ok ? E.v[wb][ok ? nb : 0][up ? 1 : 0][ok ? r : 0][1][w] : 0ull;
-        V1[c][w] = (up || dn) ? e1 : V1[c][w];
-        V2[c][w] = (up || dn) ? e2 : V2[c][w];
+        const uint64_t n1 = (up || dn) ? e1 : V1[c][w];
+        const uint64_t n2 = (up || dn) ? e2 : V2[c][w];
+        if (__ballot(n1 != V1[c][w] || n2 != V2[c][w]) != 0) ch |= 1u << (c * WD + w);
+        V1[c][w] = n1;
+        V2[c][w] = n2;
       }
     }
+    // cells that changed at the last level or took new halo rows
+    act |= spread(ch);
   }
   if (esc_sides) {
     // does seed 1's component touch an open window edge?  Loads outside the
@@ -1201,6 +1241,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
 #pragma unroll
     for (int w = 0; w < WD; w++) {
       const uint64_t K = V1[c][w] & ~NK[c][w];
+      if (__ballot(own && K != 0) == 0) continue;  // wave-uniform: nothing moves here
       const int col = 64 * w + lane;
       for (int rr = kHalo; rr < 64 - kHalo; rr++) {
         const int y = row0 + rr;
