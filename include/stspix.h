@@ -238,6 +238,40 @@ enum { STSP_SWEEP_SHAPES = 6 };
 int stsp_batch_tune_sweeps(stsp_batch* batch, int nlaunch, hipStream_t stream,
                            float* us_per_sweep, int* best);
 
+/* The per-step label ops of the st_spix.prop pipeline (the reference's
+ * prop_cuda extension), standalone; the fused stsp_prop has its own shift
+ * and fill (the BIST forms).  Device pointers; asynchronous on `stream`
+ * unless stated.
+ *
+ * prop_cuda.shift_labels (csrc/prop/shift_labels.cu:81-126,
+ * run_shift_labels(spix, flow) -> (shifted_spix, counts)): spix
+ * [nbatch][H][W] int32; flow [nbatch][nspix][2] int32 as (dx, dy) per label.
+ * Every pixel moves by its label's flow; each destination counts its
+ * arrivals (counts) and keeps the largest arriving label (shifted; -1 where
+ * none).  Destinations outside the image are dropped; pixels whose label is
+ * outside [0, nspix) do not move (the reference reads past the flow array
+ * there). */
+int stsp_shift_labels(const int* spix, const int* flow, int nbatch, int height, int width,
+                      int nspix, int* shifted, int* counts, hipStream_t stream);
+
+/* prop_cuda.fill_missing (csrc/prop/fill_missing.cu:317-354,
+ * run_fill_missing(spix, centers, missing, break_iter) -> filled): fills, in
+ * place, the -1 pixels of seg [H][W] listed in missing[nmissing] (flat
+ * indices; entries outside the image are ignored).  Each round marks the
+ * listed -1 pixels with a labelled 4-neighbour and gives them, in four
+ * checkerboard passes, the neighbouring label whose centre
+ * centers[ncenters][2] (float x, y) is nearest in coordinates normalised by
+ * width and height; labels >= ncenters are not candidates.  Rounds stop
+ * when a round counts no -1 among the listed pixels, counts as many as the
+ * round before (the first round compares with 1), or after break_iter
+ * rounds (break_iter <= 0: no cap).  Blocks until done; *rounds_out
+ * receives the rounds run and *negatives_out the listed -1 entries left
+ * (the reference prints "negative spix exist." then).  Allocates a
+ * workspace of H*W + 2.3 KB bytes on the stream. */
+int stsp_fill_missing(int* seg, const float* centers, int ncenters, const int* missing,
+                      int nmissing, int height, int width, int break_iter, int* rounds_out,
+                      int* negatives_out, hipStream_t stream);
+
 const char* stsp_status_string(int status);
 
 #ifdef __cplusplus

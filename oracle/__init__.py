@@ -189,6 +189,32 @@ def fill_missing(seg, centers, break_iter=1000):
     return seg, rc
 
 
+def prop_shift_labels(spix, flow):
+    """prop_cuda.shift_labels for one image: spix (H, W) int32, flow
+    (nspix, 2) int32 as (dx, dy).  Returns (shifted, counts)."""
+    spix = np.ascontiguousarray(spix, np.int32)
+    flow = np.ascontiguousarray(flow, np.int32)
+    H, W = spix.shape
+    shifted = np.empty((H, W), np.int32)
+    counts = np.empty((H, W), np.int32)
+    lib().orc_prop_shift_labels(_p(spix), _p(flow), flow.shape[0], H, W, _p(shifted), _p(counts))
+    return shifted, counts
+
+
+def prop_fill_missing(seg, centers, missing, break_iter):
+    """prop_cuda.fill_missing for one image: seg (H, W) int32, centers
+    (n, 2) float32, missing flat indices.  Returns (filled, rounds,
+    negatives left)."""
+    seg = np.ascontiguousarray(seg, np.int32).copy()
+    centers = np.ascontiguousarray(centers, np.float32)
+    missing = np.ascontiguousarray(np.asarray(missing).ravel(), np.int32)
+    H, W = seg.shape
+    neg = ctypes.c_int()
+    rounds = lib().orc_prop_fill_missing(_p(seg), _p(centers), centers.shape[0], _p(missing),
+                                         missing.size, H, W, int(break_iter), ctypes.byref(neg))
+    return seg, rounds, neg.value
+
+
 def invalidate_disconnected(seg, nspix):
     seg = np.ascontiguousarray(seg, np.int32).copy()
     H, W = seg.shape

@@ -1311,6 +1311,85 @@ int orc_fill_missing(int* seg, const double* centers, int height, int width,
   return rc;
 }
 
+/* prop_cuda.shift_labels: shift_labels_kernel (shift_labels.cu:28-62) for
+ * one image; the counts and the atomicMax of the label are order-free. */
+void orc_prop_shift_labels(const int* spix, const int* flow, int nspix, int height, int width,
+                           int* shifted, int* counts) {
+  int npix = height * width;
+  for (int i = 0; i < npix; i++) {
+    shifted[i] = -1; /* run_shift_labels: -torch::ones */
+    counts[i] = 0;
+  }
+  for (int i = 0; i < npix; i++) {
+    int L = spix[i];
+    if (L < 0 || L >= nspix) continue;
+    int hd = i / width + flow[2 * L + 1], wd = i % width + flow[2 * L];
+    if (hd < 0 || hd >= height || wd < 0 || wd >= width) continue;
+    int d = hd * width + wd;
+    counts[d] += 1;
+    if (L > shifted[d]) shifted[d] = L;
+  }
+}
+
+/* prop_cuda.fill_missing: the host loop of fill_missing (fill_missing.cu:
+ * 49-110), find_border_along_missing (276-314) over the listed pixels and
+ * four checkerboard passes of update_missing_seg_nn (142-273) with the float
+ * isotropic_space (127-140). */
+int orc_prop_fill_missing(int* seg, const float* centers, int ncenters, const int* missing,
+                          int nmissing, int height, int width, int break_iter, int* negatives) {
+  int npix = height * width;
+  unsigned char* border = (unsigned char*)calloc((size_t)npix, 1);
+  int prev_neg = 1, num_neg = 1, iter = 0;
+  while (num_neg > 0) {
+    if (break_iter > 0 && iter >= break_iter) break;
+    memset(border, 0, (size_t)npix);
+    num_neg = 0;
+    for (int j = 0; j < nmissing; j++) {
+      int idx = missing[j];
+      if (idx < 0 || idx >= npix) continue; /* outside the image: ignored */
+      int x = idx % width, y = idx / width;
+      int C = seg[idx];
+      int N = y > 0 ? seg[idx - width] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
+      int S = y < height - 1 ? seg[idx + width] : -1, E = x < width - 1 ? seg[idx + 1] : -1;
+      int valid = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
+      if (valid && C < 0) border[idx] = 1;
+      if (C < 0) num_neg++;
+    }
+    for (int xm = 0; xm < 2; xm++)
+      for (int ym = 0; ym < 2; ym++)
+        for (int idx = 0; idx < npix; idx++) {
+          int x = idx % width, y = idx / width;
+          if (x % 2 != xm || y % 2 != ym || !border[idx]) continue;
+          int N = y > 0 ? seg[idx - width] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
+          int E = x < width - 1 ? seg[idx + 1] : -1, S = y < height - 1 ? seg[idx + width] : -1;
+          float best = -999999.0f, lab = -1.0f; /* res_max (x, y) of float2 */
+          int cand[4] = {N, S, Wl, E};
+          for (int j = 0; j < 4; j++) {
+            int L = cand[j], dup = 0;
+            for (int q = 0; q < j; q++) dup |= cand[q] == L;
+            if (L < 0 || dup || L >= ncenters) continue;
+            float dx = (1.0f * (float)x - centers[2 * L]) / (1.0f * (float)width);
+            float dy = (1.0f * (float)y - centers[2 * L + 1]) / (1.0f * (float)height);
+            float sim = -dx * dx - dy * dy;
+            if (sim > best) {
+              best = sim;
+              lab = (float)L;
+            }
+          }
+          seg[idx] = (int)lab;
+        }
+    iter++;
+    if (iter > 0 && num_neg == prev_neg) break; /* "the border won't shrink" */
+    prev_neg = num_neg;
+  }
+  int left = 0;
+  for (int j = 0; j < nmissing; j++)
+    left += missing[j] >= 0 && missing[j] < npix && seg[missing[j]] < 0;
+  if (negatives) *negatives = left;
+  free(border);
+  return iter;
+}
+
 /* run_invalidate_disconnected, split_disconnected.cu:33-185 (rule R6) */
 void orc_invalidate_disconnected(int* seg, int height, int width, int nspix) {
   int npix = height * width;

@@ -93,6 +93,21 @@ int orc_fill_missing(int* seg, const double* centers, int height, int width,
                      int break_iter);
 void orc_invalidate_disconnected(int* seg, int height, int width, int nspix);
 
+/* The per-step ops of the st_spix.prop pipeline (prop_cuda). */
+/* shift_labels, csrc/prop/shift_labels.cu:28-62 and 81-126: pixels move by
+ * their label's integer flow flow[nspix][2] = (dx, dy); each destination
+ * counts arrivals and keeps the largest label (-1 where none).  Labels
+ * outside [0, nspix) do not move. */
+void orc_prop_shift_labels(const int* spix, const int* flow, int nspix, int height, int width,
+                           int* shifted, int* counts);
+/* fill_missing, csrc/prop/fill_missing.cu:49-110, 127-140, 142-273, 276-314:
+ * fills the -1 pixels listed in missing[nmissing] from their 4-neighbours by
+ * the nearest float centre centers[ncenters][2]; labels >= ncenters are not
+ * candidates.  Returns the number of rounds run; *negatives receives the
+ * listed -1 entries left. */
+int orc_prop_fill_missing(int* seg, const float* centers, int ncenters, const int* missing,
+                          int nmissing, int height, int width, int break_iter, int* negatives);
+
 /* Single steps of the restatement, exposed so the tests can pin each one
  * against closed forms, library routines and brute force. */
 /* calc_joint (update_seg.cu:194-259) of one pixel for one candidate label */

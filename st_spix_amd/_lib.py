@@ -64,7 +64,7 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_set_config", "stsp_bass_capacity", "stsp_prop_capacity", "stsp_bass",
            "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
            "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
-           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_batch_tune_sweeps", "stsp_prop_batch", "stsp_bass_batch",
+           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_batch_tune_sweeps", "stsp_shift_labels", "stsp_fill_missing", "stsp_prop_batch", "stsp_bass_batch",
            "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb", "stsp_ctx_set_split_search",
            "stsp_ctx_split_search_counts", "stsp_bass_noalloc", "stsp_bass_noalloc_scratch_bytes",
            "stsp_prop_ex", "stsp_ctx_device_bytes"]
@@ -111,6 +111,8 @@ def lib() -> ctypes.CDLL:
         L.stsp_batch_time_sweeps.argtypes = [vp, i, vp, ctypes.POINTER(ctypes.c_float)]
         L.stsp_batch_tune_sweeps.argtypes = [vp, i, vp, ctypes.POINTER(ctypes.c_float),
                                              ctypes.POINTER(i)]
+        L.stsp_shift_labels.argtypes = [vp, vp, i, i, i, i, vp, vp, vp]
+        L.stsp_fill_missing.argtypes = [vp, vp, i, vp, i, i, i, i, pi, pi, vp]
         L.stsp_prop_batch.argtypes = [vp, ctypes.POINTER(vp), i, ctypes.POINTER(vp),
                                       ctypes.POINTER(vp), ctypes.POINTER(vp),
                                       ctypes.POINTER(ParamsStruct), pi, ctypes.POINTER(vp),

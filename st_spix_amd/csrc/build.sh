@@ -10,5 +10,5 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -Wno-unused-command-line-argument \
   -o "$OUT/libstspix.so.tmp" \
-  "$HERE/k_bass.hip" "$HERE/k_batch.hip" "$HERE/k_prop.hip" "$HERE/k_color.hip" "$HERE/host.hip"
+  "$HERE/k_bass.hip" "$HERE/k_batch.hip" "$HERE/k_prop.hip" "$HERE/k_color.hip" "$HERE/k_steps.hip" "$HERE/host.hip"
 mv "$OUT/libstspix.so.tmp" "$OUT/libstspix.so"

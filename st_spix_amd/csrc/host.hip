@@ -387,13 +387,14 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipMalloc((void**)&bh->d_pro, sizeof(ProFrame) * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
   HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
-  HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf * kNegSub));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro, sizeof(ProFrame) * maxf, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro_off, sizeof(int) * kProKinds * (kProMax + 1),
                         hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_fillcnt, sizeof(int) * kFillChunk * maxf,
                         hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf * kNegSub,
+                        hipHostMallocDefault));
   const size_t live = sizeof(int) * (kProKinds * (kProMax + 1) + kProMax);
   HIP_TRY(hipMalloc((void**)&bh->d_live, live));
   HIP_TRY(hipHostMalloc((void**)&bh->h_live, live, hipHostMallocDefault));
@@ -1106,14 +1107,16 @@ static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* co
     if (filled_out && filled_out[i])
       HIP_TRY(hipMemcpyAsync(filled_out[i], cs[i]->seg, bytes, hipMemcpyDeviceToDevice, s));
   }
-  HIP_TRY(hipMemcpyAsync(bh->h_negs, p.negs, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(bh->h_negs, p.negs, sizeof(int) * n * kNegSub, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = cs[i];
     const stsp_config& cfg = c->cfg;
     const int np = nprev[i];
     // adaptive iteration count (pyapi.cu:139-149): holes of the shifted map
-    float iperc = (float)(bh->h_negs[i] / (1.0 * c->npix));
+    long long holes = 0;
+    for (int j = 0; j < kNegSub; j++) holes += bh->h_negs[i * kNegSub + j];
+    float iperc = (float)(holes / (1.0 * c->npix));
     int adapt = iperc > 0.20f ? 12 : (iperc < 0.01f ? 4 : 8);
     if (niters_state) niters_state[i] = adapt;
     c->img = imgs[i];

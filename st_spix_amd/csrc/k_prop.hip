@@ -68,10 +68,8 @@ int pro_blocks(int kind, int W, int H, int np, int nbuf) {
 // ---------------------------------------------------------------------------
 __global__ void kp_clear(Pro p) {
   PRO_LAB
-  if (k == 0) {
-    p.negs[f] = 0;
-    *F.max_spix = F.np - 1;
-  }
+  if (k < kNegSub) p.negs[f * kNegSub + k] = 0;
+  if (k == 0) *F.max_spix = F.np - 1;
   if (k >= F.np) return;
   F.fsum[2 * k] = F.fsum[2 * k + 1] = 0;
   F.counts_prev[k] = 0;
@@ -167,7 +165,9 @@ __global__ void kp_count_labels(Pro p, int which) {
 // holes of the shifted map (the adaptive iteration count, pyapi.cu:139-149)
 __global__ void kp_count_holes(Pro p) {
   PRO_PIX
-  block_count_add(t < F.npix && F.shifted[t] < 0, p.negs + f);
+  // consecutive blocks of a frame take consecutive partial counters
+  block_count_add(t < F.npix && F.shifted[t] < 0,
+                  p.negs + f * kNegSub + ((t / kBlock) & (kNegSub - 1)));
 }
 
 void pro_shift(const Pro& p, const int* total, hipStream_t s) {

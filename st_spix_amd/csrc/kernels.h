@@ -73,6 +73,10 @@ struct ProFrame {
   unsigned long long* best;
 };
 enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProKinds };
+// partial hole counters per frame: the counting blocks of a frame spread their
+// atomics over kNegSub addresses instead of one
+constexpr int kNegSub = 64;
+
 struct Pro {
   const ProFrame* fr;  // device table
   const int* off;      // [kProKinds][kProMax + 1] over the frames launched
@@ -80,7 +84,8 @@ struct Pro {
   const int* fmap;     // launched frame i is table frame fmap[i] (null: i itself)
   int nall;            // frames in the table
   int* fillcnt;        // [kFillMax][nall]: holes at the start of each round
-  int* negs;           // [nall]: holes of the shifted map
+  int* negs;           // [nall * kNegSub]: holes of the shifted map, kNegSub partial
+                       // counters per frame (summed by the host)
 };
 // host side: block counts per kind for a frame
 int pro_blocks(int kind, int W, int H, int np, int nbuf);
