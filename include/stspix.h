@@ -99,11 +99,24 @@ int stsp_ctx_set_tuning(stsp_ctx* ctx, int sweep_shape);
  * search is a level-synchronous flood fill over the whole frame,
  * split_merge_prop.cu:277-323).  Bounding boxes up to wave_box x wave_box
  * pixels (1..128) are searched by one wavefront; larger ones by a workgroup
- * in a window of at most win_rows x win_cols pixels (1..448, 1..384) around
+ * in a window of at most win_rows x win_cols pixels (1..896, 1..512) around
  * the first seed, then over the whole box if the seed's connected component
  * leaves the window.  Results are identical for every setting; only speed
- * differs.  Defaults: 128, 448, 384. */
+ * differs.  Defaults: 128, 896, 512. */
 int stsp_ctx_set_split_search(stsp_ctx* ctx, int wave_box, int win_rows, int win_cols);
+
+/* Shape (inverse-Wishart) term in the split Hastings ratio, off by default.
+ * The reference computes it (marginal_likelihood_shape_p,
+ * split_merge_prop.cu:1596-1621) but leaves its launches commented out
+ * (split_merge_prop.cu:397-415), so the live reference ratio is the
+ * appearance one alone; on = 1 adds, in FP64, for the two parts k, s of a
+ * label against the whole f, L(k) + L(s) - L(f) with
+ *   L = nu log det Psi - (nu + n) log det Sigma_hat
+ * (Sigma_hat the sample covariance of the part's n pixel positions, Psi
+ * the label's prior_sigma_shape, nu its prior count; L = -1e7 when
+ * det Sigma_hat < 0.001; no term when det Psi < 0.001).  An extension:
+ * results then differ from the reference's. */
+int stsp_ctx_set_shape_marginal(stsp_ctx* ctx, int on);
 
 /* Cumulative number of split-proposal searches by path since the context was
  * created: [0] one wavefront, [1] workgroup over the whole box, [2] workgroup

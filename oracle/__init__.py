@@ -41,7 +41,8 @@ class Cfg(ctypes.Structure):
                 ("sigma2_app", ctypes.c_float), ("potts", ctypes.c_float),
                 ("alpha_hastings", ctypes.c_float), ("split_alpha", ctypes.c_float),
                 ("merge_alpha", ctypes.c_float), ("thresh_relabel", ctypes.c_float),
-                ("thresh_new", ctypes.c_float), ("target_nspix", ctypes.c_int)]
+                ("thresh_new", ctypes.c_float), ("target_nspix", ctypes.c_int),
+                ("shape_marginal", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
@@ -60,6 +61,9 @@ def lib():
         L = ctypes.CDLL(_SO)
         L.orc_det_log.restype = ctypes.c_double
         L.orc_det_log.argtypes = [ctypes.c_double]
+        L.orc_shape_marginal.restype = ctypes.c_double
+        L.orc_shape_marginal.argtypes = [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_double]
         _lib = L
     return _lib
 
@@ -72,7 +76,7 @@ def make_cfg(**kw) -> Cfg:
     d = dict(niters=500, niters_seg=4, sm_start=0, sp_size=25, sm_period=4,
              prop_niters=0, sigma2_app=8.1e-5, potts=10.0,
              alpha_hastings=float(np.log(0.5)), split_alpha=0.0, merge_alpha=0.0,
-             thresh_relabel=1e-5, thresh_new=1e-2, target_nspix=0)
+             thresh_relabel=1e-5, thresh_new=1e-2, target_nspix=0, shape_marginal=0)
     d.update(kw)
     return Cfg(**d)
 
@@ -187,6 +191,13 @@ def fill_missing(seg, centers, break_iter=1000):
     rc = lib().orc_fill_missing(_p(seg), _p(np.ascontiguousarray(centers, np.float64)),
                                 H, W, break_iter)
     return seg, rc
+
+
+def shape_marginal(n, sums, psi, nu):
+    """stsp_shape_marginal: the shape log marginal of one part."""
+    sums = np.ascontiguousarray(sums, np.int64)
+    psi = np.ascontiguousarray(psi, np.float64)
+    return lib().orc_shape_marginal(int(n), sums.ctypes.data, psi.ctypes.data, float(nu))
 
 
 def prop_shift_labels(spix, flow):

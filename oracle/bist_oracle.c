@@ -63,6 +63,11 @@
 #define FX_FLOW 16777216.0    /* 2^24 */
 
 double orc_det_log(double x) { return stsp_det_log(x); }
+double orc_shape_marginal(long long n, const long long* sums, const double* psi, double nu) {
+  int64_t s5[5];
+  for (int c = 0; c < 5; c++) s5[c] = sums[c];
+  return stsp_shape_marginal(n, s5, psi, nu);
+}
 
 /* ------------------------------------------------------------------ */
 /* simple-point table, seg_utils.h:15-27 (a lookup table that must be  */
@@ -534,6 +539,7 @@ typedef struct {
   int count, count_f;
   int64_t S[3], Q[3];
   int ninvalid;
+  int64_t X[5]; /* position sums x, y, xx, xy, yy (the shape term) */
   double b_n[3], b_n_f[3];
   int has_bn;
   double hasting;
@@ -626,6 +632,7 @@ typedef struct {
   int* pairs;
   osm* sm;
   int nsm;
+  int shape_marg; /* add the shape (inverse-Wishart) term to the split ratio */
 } sm_work;
 
 /* the two-seed split proposal shared by both paths: returns the half map */
@@ -662,9 +669,14 @@ static void sm_accumulate(const float* img, const int* lab, const int* shifted, 
       s->S[c] += fx_color(img[3 * t + c]);
       s->Q[c] += fx_square(img[3 * t + c]);
     }
+    int64_t x = t % width, y = t / width;
+    s->X[0] += x;
+    s->X[1] += y;
+    s->X[2] += x * x;
+    s->X[3] += x * y;
+    s->X[4] += y * y;
     if (shifted && shifted[t] < 0) s->ninvalid += 1;
   }
-  (void)width;
 }
 
 /* CudaCalcSplitCandidate_orig / _p: split_merge_orig.cu:202-420,
@@ -692,6 +704,9 @@ static void split_eval(osp* P, const ostat* st, int nbuf, int max_spix, const ni
     double ms = nig_marginal(nc, (double)count_s, bs);
     double mf = nig_marginal(nc, (double)count_f, bf);
     double h = mk + ms - mf + (double)split_alpha;
+    if (w->shape_marg)  /* extension, default off: the reference leaves it unused */
+      h += stsp_split_shape_term(count_k, w->sm[k].X, count_s, w->sm[s].X,
+                                 P[k].prior_sigma_shape, (double)P[k].prior_count);
     if (prop_mode) {
       /* update_split_flag_p, split_merge_prop.cu:3053-3068 */
       int is_prop = P[k].prop;
@@ -1060,6 +1075,7 @@ static sm_work sm_alloc(int npix, int nbuf) {
   w.pairs = (int*)malloc(sizeof(int) * 2 * (size_t)(nbuf + 2));
   w.nsm = 2 * nbuf + 2;
   w.sm = (osm*)calloc((size_t)w.nsm, sizeof(osm));
+  w.shape_marg = 0;
   return w;
 }
 static void sm_free(sm_work* w) {
@@ -1110,6 +1126,7 @@ int orc_bass(const float* img, int height, int width, const orc_cfg* cfg, int* s
   ostat* st = (ostat*)calloc((size_t)nbuf, sizeof(ostat));
   unsigned char* border = (unsigned char*)calloc((size_t)npix, 1);
   sm_work w = sm_alloc(npix, nbuf);
+  w.shape_marg = cfg->shape_marginal;
   sweep_buf sb = sweep_alloc(nbuf);
   /* mark_active_contiguous, init_sparams.cu:195-231 */
   for (int k = 0; k < nbuf; k++) {
@@ -1600,6 +1617,7 @@ int orc_prop(const float* img, const float* flow, int height, int width,
   ostat* st = (ostat*)calloc((size_t)nbuf, sizeof(ostat));
   unsigned char* border = (unsigned char*)calloc((size_t)npix, 1);
   sm_work w = sm_alloc(npix, nbuf);
+  w.shape_marg = cfg->shape_marginal;
   sweep_buf sb = sweep_alloc(nbuf);
   /* get_vectors_as_params + write_params, sparams_io.cu:123-140, 352-501 */
   for (int ix = 0; ix < nspix; ix++) {

@@ -33,6 +33,8 @@ typedef struct {
   float thresh_relabel;
   float thresh_new;
   int target_nspix;    /* >0: steer split/merge alphas to this count (bass.cu:65-105) */
+  int shape_marginal;  /* 1: add the shape (inverse-Wishart) term to the split ratio
+                          (an extension; the reference leaves it unused, 0) */
 } orc_cfg;
 
 /* SoA parameter set, field-for-field the reference SuperpixelParams
@@ -107,6 +109,10 @@ void orc_prop_shift_labels(const int* spix, const int* flow, int nspix, int heig
  * listed -1 entries left. */
 int orc_prop_fill_missing(int* seg, const float* centers, int ncenters, const int* missing,
                           int nmissing, int height, int width, int break_iter, int* negatives);
+
+/* stsp_shape_marginal of one part (n pixels, position sums x, y, xx, xy,
+ * yy), exposed for the closed-form pin */
+double orc_shape_marginal(long long n, const long long* sums, const double* psi, double nu);
 
 /* Single steps of the restatement, exposed so the tests can pin each one
  * against closed forms, library routines and brute force. */

@@ -64,7 +64,7 @@ EXPORTS = ["stsp_default_config", "stsp_ctx_create", "stsp_ctx_destroy",
            "stsp_ctx_set_config", "stsp_bass_capacity", "stsp_prop_capacity", "stsp_bass",
            "stsp_prop", "stsp_marked_video", "stsp_status_string", "stsp_ctx_enable_timing",
            "stsp_ctx_kernel_time", "stsp_ctx_set_tuning", "stsp_batch_create",
-           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_batch_tune_sweeps", "stsp_shift_labels", "stsp_fill_missing", "stsp_prop_batch", "stsp_bass_batch",
+           "stsp_batch_destroy", "stsp_batch_use_graph", "stsp_batch_time_sweeps", "stsp_batch_tune_sweeps", "stsp_shift_labels", "stsp_fill_missing", "stsp_ctx_set_shape_marginal", "stsp_prop_batch", "stsp_bass_batch",
            "stsp_rescale_image", "stsp_rgb2lab", "stsp_lab2rgb", "stsp_ctx_set_split_search",
            "stsp_ctx_split_search_counts", "stsp_bass_noalloc", "stsp_bass_noalloc_scratch_bytes",
            "stsp_prop_ex", "stsp_ctx_device_bytes"]
@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
                                            ctypes.POINTER(ctypes.c_longlong)]
         L.stsp_ctx_set_tuning.argtypes = [vp, i]
         L.stsp_ctx_set_split_search.argtypes = [vp, i, i, i]
+        L.stsp_ctx_set_shape_marginal.argtypes = [vp, i]
         L.stsp_ctx_split_search_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.stsp_bass_noalloc.argtypes = [ctypes.POINTER(Config), i, i, vp, vp,
                                         ctypes.POINTER(ParamsStruct), pi, vp]

@@ -75,8 +75,14 @@ class Context:
                    "stsp_ctx_set_tuning")
         self.sweep_shape = int(sweep_shape)
 
-    def set_split_search(self, wave_box: int = 128, win_rows: int = 448,
-                         win_cols: int = 384) -> None:
+    def set_shape_marginal(self, on: bool) -> None:
+        """Adds the shape (inverse-Wishart) term to the split ratio
+        (stsp_ctx_set_shape_marginal; an extension, off by default)."""
+        _lib.check(_lib.lib().stsp_ctx_set_shape_marginal(self.handle, int(bool(on))),
+                   "stsp_ctx_set_shape_marginal")
+
+    def set_split_search(self, wave_box: int = 128, win_rows: int = 896,
+                         win_cols: int = 512) -> None:
         """Limits of the split proposal's search paths (stsp_ctx_set_split_search);
         results are the same for every setting."""
         _lib.check(_lib.lib().stsp_ctx_set_split_search(self.handle, int(wave_box),

@@ -33,6 +33,7 @@ struct FrameDesc {
   // split proposal: boxes up to wave_box x wave_box are searched by one wave;
   // larger boxes by a workgroup in a window of at most win_h x win_w pixels
   int wave_box, win_h, win_w;
+  int shape_marg;  // add the shape term to the split ratio
   unsigned long long* search_count;  // cumulative searches per path (kSearchPaths)
   // relabel
   const float* prev_prior_mu_app;

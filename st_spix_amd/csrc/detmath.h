@@ -84,4 +84,45 @@ static inline STSP_HD double stsp_det_log(double x) {
   return de * ln2_hi + (logm + de * ln2_lo);
 }
 
+/*
+ * Shape (inverse-Wishart) log marginal of one part of a label, the term of
+ * marginal_likelihood_shape_p (split_merge_prop.cu:1596-1621):
+ *   nu * log det Psi - (nu + n) * log det Sigma_hat
+ * with Sigma_hat the sample covariance (divided by n) of the part's n pixel
+ * positions from their exact integer sums, Psi the prior scatter
+ * (xx, xy, yy) and nu the prior count; -1e7 when det Sigma_hat < 0.001, as
+ * the reference.  The lgamma terms the reference computes but does not use
+ * are left out.  Shared by the device kernels and the CPU checker so both
+ * give the same bits.
+ */
+static inline STSP_HD double stsp_shape_marginal(int64_t n, const int64_t s[5], const double psi[3],
+                                                 double nu) {
+  const double dn = (double)n;
+  const double mx = (double)s[0] / dn, my = (double)s[1] / dn;
+  const double cxx = (double)s[2] / dn - mx * mx;
+  const double cxy = (double)s[3] / dn - mx * my;
+  const double cyy = (double)s[4] / dn - my * my;
+  double det_est = cxx * cyy - cxy * cxy;
+  det_est = det_est < 0.0 ? -det_est : det_est;
+  if (det_est < 0.001) return -10000000.0;
+  double det_pr = psi[0] * psi[2] - psi[1] * psi[1];
+  det_pr = det_pr < 0.0 ? -det_pr : det_pr;
+  return nu * stsp_det_log(det_pr) - (nu + dn) * stsp_det_log(det_est);
+}
+
+/* The split's shape term: parts k and s of label k against the whole (the
+ * sums of both parts); 0 when the prior scatter is degenerate
+ * (det Psi < 0.001, where the reference asserts). */
+static inline STSP_HD double stsp_split_shape_term(int64_t nk, const int64_t sk[5], int64_t ns,
+                                                   const int64_t ss[5], const double psi[3],
+                                                   double nu) {
+  double det_pr = psi[0] * psi[2] - psi[1] * psi[1];
+  det_pr = det_pr < 0.0 ? -det_pr : det_pr;
+  if (det_pr < 0.001) return 0.0;
+  int64_t sf[5];
+  for (int c = 0; c < 5; c++) sf[c] = sk[c] + ss[c];
+  return stsp_shape_marginal(nk, sk, psi, nu) + stsp_shape_marginal(ns, ss, psi, nu) -
+         stsp_shape_marginal(nk + ns, sf, psi, nu);
+}
+
 #endif /* STSP_DETMATH_H */

@@ -188,6 +188,7 @@ struct stsp_ctx {
   int *max_spix, *total, *num_new, *counter;
   unsigned long long* search_count;  // kSearchPaths cumulative counters
   int wave_box = kWaveBoxMax, win_h = 0, win_w = 0;  // split search limits (0: widest)
+  int shape_marginal = 0;  // the shape term in the split ratio (an extension)
   // workspace carved from the static scratch buffer: nothing is allocated,
   // the side streams are not used and the capacity is fixed
   bool noalloc = false;
@@ -585,6 +586,7 @@ static FrameDesc make_desc(stsp_ctx* c) {
   d.total = c->total;
   d.counter = c->counter;
   d.wave_box = c->wave_box;
+  d.shape_marg = c->shape_marginal;
   d.win_h = c->win_h > 0 ? c->win_h : kSplitWinRows;
   d.win_w = c->win_w > 0 ? c->win_w : kSplitWinCols;
   d.search_count = c->search_count;
@@ -1268,6 +1270,7 @@ static stsp_ctx* noalloc_context(NoAllocDevice& d, const stsp_config& cfg, int H
   c->sweep_shape = 4;
   c->wave_box = kWaveBoxMax;
   c->win_h = c->win_w = 0;
+  c->shape_marginal = 0;
   c->nprev = 0;
   c->host = d.host;
   char* base = d.ptr;
@@ -1390,6 +1393,12 @@ int stsp_ctx_set_split_search(stsp_ctx* c, int wave_box, int win_rows, int win_c
   c->wave_box = wave_box;
   c->win_h = win_rows;
   c->win_w = win_cols;
+  return STSP_OK;
+}
+
+int stsp_ctx_set_shape_marginal(stsp_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return STSP_ERR_ARG;
+  c->shape_marginal = on;
   return STSP_OK;
 }
 

@@ -1698,6 +1698,9 @@ __global__ void kb_split_eval(Batch b) {
   }
   double h = nig_marg(nc, (double)count_k, bk) + nig_marg(nc, (double)count_s, bs) -
              nig_marg(nc, (double)count_f, bf) + (double)F.split_alpha;
+  if (F.shape_marg)  // extension, default off: the reference leaves this term unused
+    h += stsp_split_shape_term(count_k, sst.X + 5 * k, count_s, sst.X + 5 * s,
+                               P.prior_sigma_shape + 3 * k, (double)P.prior_count[k]);
   if (F.prop_mode) {
     int is_prop = P.prop[k];
     int nk = sst.ninvalid[k], ns = sst.ninvalid[s];

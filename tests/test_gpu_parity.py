@@ -255,3 +255,28 @@ def test_config4_full_size_sm_every_2(orc):
         assert ns == ns_r
         assert_labels(spix, seg_r, f"frame{t}")
         assert_params(params, p_r, f"frame{t}")
+
+
+@pytest.mark.parametrize("H,W,sp", [(60, 90, 8), (120, 200, 10)])
+def test_shape_marginal_extension_matches_oracle(orc, H, W, sp):
+    """The default-off shape term (stsp_ctx_set_shape_marginal) on: BASS and
+    two propagated frames equal the oracle's with the term on, bit for bit
+    (superpixel sizes at which the term is live)."""
+    from st_spix_amd import core
+    vid, flows = synth.voronoi_video(3, H, W, nregions=10, vmax=3.0, seed=H)
+    kw = dict(niters=16, sp_size=sp, potts=10.0, prop_niters=12)
+    ocfg = orc_cfg(orc, shape_marginal=1, **kw)
+    cfg = core.make_config(**kw)
+    ctx = core.Context(H, W, cfg)
+    ctx.set_shape_marginal(True)
+    spix, params = core.bass_frame(cuda(vid[0]), cfg, ctx=ctx)
+    seg_r, p_r = orc.bass(vid[0], ocfg)
+    assert_labels(spix, seg_r, "frame 0")
+    assert_params(params, p_r, "frame 0")
+    for t in range(1, 3):
+        spix, params, _ = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg,
+                                          ctx=ctx)
+        seg_r, p_r, _ = orc.prop(vid[t], flows[t - 1], seg_r, p_r, ocfg)
+        assert_labels(spix, seg_r, f"frame {t}")
+        assert_params(params, p_r, f"frame {t}")
+    ctx.close()

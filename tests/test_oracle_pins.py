@@ -463,3 +463,56 @@ def test_split_eval_matches_bass_split(orc):
         halves, _, _, _ = orc.split_eval(img, seg, K, direction, 8.1e-5)
         h_r, _, _ = split_ref(img, seg, K, direction, 8.1e-5)
         np.testing.assert_array_equal(halves, h_r)
+
+
+# ------------------------------------------------- shape (inverse-Wishart) term
+def shape_ref(pts, psi, nu):
+    """marginal_likelihood_shape_p (split_merge_prop.cu:1596-1621) from the
+    pixel coordinates themselves: numpy's biased covariance and det."""
+    pts = np.asarray(pts, np.float64)
+    cov = np.cov(pts.T, bias=True)
+    det_est = abs(np.linalg.det(cov))
+    if det_est < 0.001:
+        return -1e7
+    det_pr = abs(psi[0] * psi[2] - psi[1] * psi[1])
+    return nu * math.log(det_pr) - (nu + len(pts)) * math.log(det_est)
+
+
+def sums_of(pts):
+    x, y = np.asarray(pts, np.int64).T
+    return np.array([x.sum(), y.sum(), (x * x).sum(), (x * y).sum(), (y * y).sum()], np.int64)
+
+
+def test_shape_marginal_closed_form(orc):
+    rng = np.random.default_rng(11)
+    for it in range(300):
+        n = int(rng.integers(3, 400))
+        c = rng.uniform(0, 1900, 2)
+        A = rng.normal(0, 1, (2, 2)) * rng.uniform(1, 30)
+        pts = np.unique(np.round(c + rng.normal(0, 1, (n, 2)) @ A).astype(np.int64), axis=0)
+        psi = np.array([rng.uniform(5, 400), rng.uniform(-20, 20), rng.uniform(5, 400)])
+        nu = float(rng.uniform(1, 500))
+        got = orc.shape_marginal(len(pts), sums_of(pts), psi, nu)
+        exp = shape_ref(pts, psi, nu)
+        if exp == -1e7:
+            assert got == -1e7
+        else:
+            assert abs(got - exp) <= 1e-9 * max(1.0, abs(exp)), (it, got, exp)
+    # collinear pixels: det 0 -> the reference's -1e7
+    line = [(x, 2 * x + 3) for x in range(20)]
+    assert orc.shape_marginal(20, sums_of(line), np.array([30.0, 0.0, 30.0]), 10.0) == -1e7
+
+
+def test_shape_term_changes_the_split(orc):
+    """The term is live when on: BASS of small superpixels (sp 8, prior
+    scatter det 1/64 above the 0.001 cut) gives other labels with it than
+    without it; at sp 40 the prior scatter's det (1/1600) is below the cut,
+    the term is 0 and the labels are the same."""
+    img = synth.voronoi_image(60, 90, 10, seed=3)
+    segs = {}
+    for sp in (8, 40):
+        for sm in (0, 1):
+            cfg = orc.make_cfg(niters=16, sp_size=sp, potts=10.0, shape_marginal=sm)
+            segs[sp, sm] = orc.bass(img, cfg)[0]
+    assert not np.array_equal(segs[8, 0], segs[8, 1])
+    np.testing.assert_array_equal(segs[40, 0], segs[40, 1])
