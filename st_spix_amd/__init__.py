@@ -3,7 +3,11 @@
 Drop-in for the reference's ``bist_cuda`` module (bist_forward,
 get_marked_video; st_spix/csrc/pybind_bist.cpp:16-17) and for the per-frame
 ``st_spix.bass`` / ``st_spix.prop`` entry points, computed by hand-written
-HIP kernels in ``_lib/libstspix.so``.
+HIP kernels in ``_lib/libstspix.so``.  ``st_spix_amd.bist_cuda`` is the thin
+torch C++ extension with the reference module's two functions; the
+``bist_forward`` / ``get_marked_video`` below call it, and take the
+ctypes binding of the same library (core) only for keyword overrides of the
+constants the reference hard-codes.
 """
 from __future__ import annotations
 
@@ -28,6 +32,10 @@ def bist_forward(vid: torch.Tensor, flows: torch.Tensor, niters: int, sp_size: i
     Extra keyword arguments (niters_seg, sm_period, prop_niters, ...) override
     constants the reference hard-codes.
     """
+    if not extra:
+        return _ext().bist_forward(vid, flows, int(niters), int(sp_size), float(potts),
+                                   float(sigma2_app), float(alpha), float(split_alpha),
+                                   bool(video_mode), bool(rgb2lab))
     core._check_input(vid, "vid")
     core._check_input(flows, "flows")
     T, H, W = vid.shape[:3]
@@ -47,4 +55,17 @@ def bist_forward(vid: torch.Tensor, flows: torch.Tensor, niters: int, sp_size: i
 
 def get_marked_video(vid: torch.Tensor, spix: torch.Tensor, color) -> torch.Tensor:
     """Border overlay (reference pyapi.cu:260-303)."""
-    return core.marked_video(vid, spix, color)
+    if not torch.is_tensor(color):
+        color = torch.tensor(color, dtype=torch.float32)
+    return _ext().get_marked_video(vid, spix, color.contiguous())
+
+
+def _ext():
+    """The torch extension (bist_cuda*.so, built by csrc/build_ext.sh); a
+    missing build is an error, not a fallback."""
+    try:
+        from . import bist_cuda
+    except ImportError as e:
+        raise ImportError("st_spix_amd.bist_cuda is not built: run "
+                          "st_spix_amd/csrc/build_ext.sh (or __graft_entry__.build())") from e
+    return bist_cuda

@@ -156,6 +156,15 @@ def test_bist_forward_matches_oracle(orc):
                                     float(np.log(0.5)), 0.0, False, False)
     ref2 = orc.bist(vid, flows, orc.make_cfg(niters=20, sp_size=8, potts=10.0), video_mode=False)
     assert_labels(out2, ref2, "bist_forward image mode")
+    # the torch extension module itself, positionally as the reference's
+    # bist_cuda.bist_forward, and the keyword-override path (ctypes binding)
+    from st_spix_amd import bist_cuda
+    out3 = bist_cuda.bist_forward(cuda(vid), cuda(flows), 20, 8, 10.0, 8.1e-5,
+                                  float(np.log(0.5)), 0.0, True, False)
+    assert_labels(out3, ref, "bist_cuda.bist_forward")
+    out4 = st_spix_amd.bist_forward(cuda(vid), cuda(flows), 20, 8, 10.0, 8.1e-5,
+                                    float(np.log(0.5)), 0.0, True, False, niters_seg=4)
+    assert_labels(out4, ref, "bist_forward with an override")
 
 
 def test_marked_video():

@@ -41,3 +41,18 @@ def test_host_only_entry_points():
     assert L.stsp_ctx_create(None, 10, 10, ctypes.byref(h)) == -1
     cfg.sp_size = 0
     assert L.stsp_ctx_create(ctypes.byref(cfg), 10, 10, ctypes.byref(h)) == -1
+
+
+def test_torch_extension_loads():
+    """The thin torch C++ extension (bist_cuda, the reference module's two
+    functions) imports, links libstspix.so and checks its inputs on the host
+    the way the reference's CHECK_INPUT does."""
+    import torch
+    from st_spix_amd import bist_cuda
+    assert callable(bist_cuda.bist_forward) and callable(bist_cuda.get_marked_video)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        bist_cuda.bist_forward(torch.zeros(1, 4, 4, 3), torch.zeros(1, 4, 4, 2), 10, 8, 1.0,
+                               0.01, -0.6, 0.0, True)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        bist_cuda.get_marked_video(torch.zeros(1, 4, 4, 3), torch.zeros(1, 4, 4, dtype=torch.int32),
+                                   torch.zeros(3))
