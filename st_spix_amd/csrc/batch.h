@@ -55,6 +55,9 @@ struct FrameDesc {
 // component; the whole box after the component left the window.
 enum { kSearchWave = 0, kSearchBand, kSearchWindow, kSearchWhole, kSearchPaths };
 constexpr int kWaveBoxMax = 128;
+// label slots of a frame: labels fill 24-bit fields (work list entries
+// frame << 24 | label, the sweep's packed LDS words)
+constexpr int kLabelLimit = 1 << 24;
 constexpr int kSplitWinRows = 896, kSplitWinCols = 512;  // the band search's register rows
 constexpr int kBlockRowsMax = 448, kBlockColsMax = 384;  // boxes of the 256-thread band kernel
 
@@ -119,5 +122,9 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
 void b_merge(const Batch& b, int cur, int direction, hipStream_t s);
 void b_relabel(const Batch& b, int cur, hipStream_t s);
 void b_count_living(const Batch& b, int slot, hipStream_t s);
+// pieces of the split step that live beside the statistics kernels
+void b_sm_begin(const Batch& b, hipStream_t s);               // clear split/merge state, work lists
+void b_halves_stats(const Batch& b, int cur, hipStream_t s);  // statistics of the split halves
+void b_scan(const Batch& b, int which, hipStream_t s);        // per-frame exclusive scan
 
 }  // namespace stsp

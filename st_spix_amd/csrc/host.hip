@@ -337,6 +337,7 @@ static int carve_labels(stsp_ctx* c, char* base, int cap) {
 static int ensure_labels(stsp_ctx* c, int cap) {
   if (cap <= c->cap) return STSP_OK;
   if (c->noalloc) return STSP_ERR_TOO_LARGE;  // sized once, from the static scratch buffer
+  if (cap + cap / 4 > kLabelLimit) return STSP_ERR_TOO_LARGE;
   if (c->lab_mem) {
     if (c->done_rec) HIP_TRY(hipEventSynchronize(c->done_ev));  // the last call's work
     HIP_TRY(hipFree(c->lab_mem));
