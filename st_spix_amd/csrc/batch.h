@@ -59,7 +59,7 @@ constexpr int kWaveBoxMax = 128;
 // frame << 24 | label, the sweep's packed LDS words)
 constexpr int kLabelLimit = 1 << 24;
 constexpr int kSplitWinRows = 896, kSplitWinCols = 512;  // the band search's register rows
-constexpr int kBlockRowsMax = 448, kBlockColsMax = 384;  // boxes of the 256-thread band kernel
+constexpr int kMidRows = 448, kMidCols = 384;  // boxes of the block list (either orientation)
 
 // Launch shapes of the fused sweep: tile width, height, sweeps per launch,
 // threads.  A remainder of fewer sweeps than a shape fuses runs shape 0.

@@ -8,8 +8,9 @@ namespace stsp {
 // Labels that take part in the split (valid, non-empty, split not skipped),
 // sorted by the size of their bounding box: boxes up to wave_box x wave_box
 // (at most 128 x 128, the wave kernel's register rows) go to the wave list
-// work[0..), boxes up to 448 x 384 to the block list work[work_cap..), larger
-// ones to the big-block list work[2 work_cap..).
+// work[0..); boxes up to 448 x 384 or 384 x 448 that also fit the frame's
+// search window to the block list work[work_cap..) (searched whole by the
+// 8-wave kernel); the others to the big-block list work[2 work_cap..).
 
 __global__ void kb_split_list(Batch b) {
   LAB_PROLOGUE
@@ -23,7 +24,8 @@ __global__ void kb_split_list(Batch b) {
   if (bw <= F.wave_box && bh <= F.wave_box) {
     const int pos = atomicAdd(b.work_n + 0, 1);
     if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
-  } else if (bw <= kBlockColsMax && bh <= kBlockRowsMax) {
+  } else if (((bh <= kMidRows && bw <= kMidCols) || (bw <= kMidRows && bh <= kMidCols)) &&
+             bh <= F.win_h && bw <= F.win_w) {
     const int pos = atomicAdd(b.work_n + 2, 1);
     if (pos < b.work_cap) b.work[b.work_cap + pos] = (f << 24) | k;
   } else {
@@ -193,9 +195,10 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
 // the own rows stay exact.  Then the bands trade their outer own rows through
 // LDS (double-buffered by round parity) and one barrier ends the round.  The
 // search ends after a round in which no own row grew: the level before that
-// round was already a fixed point.  The 256-thread kernel holds up to 8 bands
-// (448 rows) of 6 words (384 columns); the 512-thread kernel for larger
-// boxes 16 bands (896 rows) of 8 words (512 columns).
+// round was already a fixed point.  The 512-thread kernel of the block list
+// holds 8 bands (448 rows) of up to 6 words (384 columns), one band per
+// wave; the 512-thread kernel of larger boxes 16 bands (896 rows) of 8 words
+// (512 columns), two bands per wave.
 //
 // A label whose box is larger is searched in a window of that size around
 // seed 1 first.  Only pixels connected to seed 1 can move, so the window's
@@ -207,12 +210,11 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
 // searched: with LDS bitsets up to kBlkWords words, else by per-pixel
 // distances in global memory over the bounding box of each level's frontier.
 constexpr int kHalo = 4, kOwn = 64 - 2 * kHalo;
-constexpr int kBlockRows = 8 * kOwn, kBlockCols = 6 * 64;  // 256 threads, 2 chunks of 4 bands
-constexpr int kBigRows = 16 * kOwn, kBigCols = 8 * 64;     // 512 threads, 2 chunks of 8 bands
+constexpr int kBigRows = 16 * kOwn, kBigCols = 8 * 64;  // 512 threads, 2 chunks of 8 bands
 static_assert(kSplitWinRows == kBigRows && kSplitWinCols == kBigCols,
               "the widest search window is the 512-thread band layout's");
-static_assert(kBlockRowsMax == kBlockRows && kBlockColsMax == kBlockCols,
-              "boxes routed to the 256-thread kernel fit its bands");
+static_assert(kMidRows == 8 * kOwn && kMidCols == 6 * 64,
+              "boxes of the block list fit 8 bands of 6 words");
 constexpr int kBlkWords = 2048;
 constexpr int kBigWords = 18 * 512;  // whole-box words of the 512-thread kernel (147 KB of LDS)
 enum { kEscLeft = 1, kEscRight = 2, kEscTop = 4, kEscBottom = 8 };
@@ -223,7 +225,6 @@ struct BandEdge {
   uint32_t g[2][NW];                          // [buf][wave]: own rows grew
   uint32_t esc;                               // seed 1's component leaves the window
 };
-typedef BandEdge<8, 6, 4> BlockEdge;
 typedef BandEdge<16, 8, 8> BigEdge;
 
 // Searches the window (bx0, by0, bw, bh) of label k.  esc_sides marks the
@@ -232,12 +233,17 @@ typedef BandEdge<16, 8, 8> BigEdge;
 // label k just outside; the halves map is then left unchanged.
 // With MLDS the label mask lives in LDS (mlds, [band][word][lane], 8 * 64 *
 // NW * RB * WD bytes, lanes on consecutive words) instead of registers.
-template <int NW, int RB, int WD, bool MLDS, class Edge>
+// With TR the box is searched transposed: the bands run over its columns
+// (a lane holds one column) and the words over its rows (bit r of word w is
+// row 64 w + r); 4-connectivity is the same either way.  Transposed
+// searches take no window (esc_sides == 0).
+template <int NW, int RB, int WD, bool MLDS, class Edge, bool TR = false>
 __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, int k, int bx0,
                                           int by0, int bw, int bh, int s1, int s2, int k2,
                                           int* __restrict__ halves, Edge& E, int esc_sides,
                                           uint64_t* __restrict__ mlds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nrow = TR ? bw : bh;  // rows of the band structure
   uint64_t M[MLDS ? 1 : RB][MLDS ? 1 : WD], V1[RB][WD], V2[RB][WD], NK[RB][WD];
   // the mask word of chunk c, word w for this lane
 #define STSP_MASK(c, w) \
@@ -252,7 +258,22 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
       uint64_t m = 0;
       V1[c][w] = V2[c][w] = NK[c][w] = 0;
       const int col = 64 * w + lane;
-      if (row0 < bh) {
+      if (TR && row0 < nrow) {
+        // lane = box column x, bit r = box row 64 w + r: each load reads 64
+        // neighbouring columns of one row
+        const int x = row0 + lane;
+        const bool xin = x >= 0 && x < bw;
+        for (int r0 = 0; r0 < 64; r0 += 16) {
+          int v[16];
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            const int y = 64 * w + r0 + j;
+            v[j] = (xin && y < bh) ? seg[(by0 + y) * W + bx0 + x] : -1;
+          }
+#pragma unroll
+          for (int j = 0; j < 16; j++) m |= (uint64_t)(v[j] == k) << (r0 + j);
+        }
+      } else if (!TR && row0 < bh) {
         for (int r0 = 0; r0 < 64; r0 += 16) {
           int v[16];  // 16 row loads in flight, then 16 ballots
 #pragma unroll
@@ -274,8 +295,11 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     }
   }
   {
-    const int y1 = s1 / W - by0, x1 = s1 % W - bx0;
-    const int y2 = s2 >= 0 ? s2 / W - by0 : -1000, x2 = s2 >= 0 ? s2 % W - bx0 : 0;
+    // seed cells in band coordinates (row y, column x)
+    const int iy1 = s1 / W - by0, ix1 = s1 % W - bx0;
+    const int iy2 = s2 >= 0 ? s2 / W - by0 : -1000, ix2 = s2 >= 0 ? s2 % W - bx0 : -1000;
+    const int y1 = TR ? ix1 : iy1, x1 = TR ? iy1 : ix1;
+    const int y2 = s2 >= 0 ? (TR ? ix2 : iy2) : -1000, x2 = s2 >= 0 ? (TR ? iy2 : ix2) : 0;
 #pragma unroll
     for (int c = 0; c < RB; c++) {
       const int y = kOwn * (NW * c + wv) - kHalo + lane;
@@ -391,7 +415,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     // cells that changed at the last level or took new halo rows
     act |= spread(ch);
   }
-  if (esc_sides) {
+  if (!TR && esc_sides) {
     // does seed 1's component touch an open window edge?  Loads outside the
     // window stay inside the label's box (esc_sides marks only such edges)
     bool esc = false;
@@ -439,6 +463,18 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     for (int w = 0; w < WD; w++) {
       const uint64_t K = V1[c][w] & ~NK[c][w];
       if (__ballot(own && K != 0) == 0) continue;  // wave-uniform: nothing moves here
+      if (TR) {
+        // this lane's own column, rows 64 w .. 64 w + 63: the lanes of one
+        // store write neighbouring columns of one row
+        const int x = row0 + lane;
+        const bool mine = own && x < bw;
+        for (int r = 0; r < 64; r++) {
+          const int y = 64 * w + r;
+          if (y >= bh) break;  // wave-uniform
+          if (mine && ((K >> r) & 1ull)) halves[(by0 + y) * W + bx0 + x] = k2;
+        }
+        continue;
+      }
       const int col = 64 * w + lane;
       for (int rr = kHalo; rr < 64 - kHalo; rr++) {
         const int y = row0 + rr;
@@ -756,8 +792,9 @@ __device__ __forceinline__ void bfs_global(const int* __restrict__ seg, int W, i
   }
 }
 
-// One workgroup of NW waves per label of work list LIST (1: boxes up to
-// 448 x 384, NW = 4; 2: larger boxes, NW = 8).
+// One workgroup of NW waves per label of work list LIST (the big-box list:
+// NW = 8, a window around seed 1 when the box exceeds it, then the whole-box
+// fallbacks).
 template <int NW, int LIST, int MAXR, int MAXC, class Edge>
 __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int direction, Edge& edge,
                                                 uint64_t* sM, uint64_t* sV1, uint64_t* sV2,
@@ -811,18 +848,8 @@ __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int dir
   escaped = bfs_bands<NW, RB, WD, (NW > 4)>(seg, W, k, wx0, wy0, ww, wh, s1, t2, k2, halves, edge, \
                                             sides, sM)
     const int rb = wh <= NW * kOwn ? 1 : 2;
-    if (NW == 4) {
-      const int wd = ww <= 128 ? 2 : (ww <= 256 ? 4 : 6);
-      if (rb == 1) {
-        if (wd == 2) STSP_BANDS(1, 2);
-        else if (wd == 4) STSP_BANDS(1, 4);
-        else STSP_BANDS(1, 6);
-      } else {
-        if (wd == 2) STSP_BANDS(2, 2);
-        else if (wd == 4) STSP_BANDS(2, 4);
-        else STSP_BANDS(2, 6);
-      }
-    } else {
+    static_assert(NW == 8, "the big-box kernel has 8 waves");
+    {
       const int wd = ww <= 256 ? 4 : 8;
       if (rb == 1) {
         if (wd == 4) STSP_BANDS(1, 4);
@@ -847,12 +874,68 @@ __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int dir
   }
 }
 
-__global__ void __launch_bounds__(256) kb_bfs_block(Batch b, int cur, int direction) {
-  __shared__ uint64_t sM[kBlkWords], sV1[kBlkWords], sV2[kBlkWords], sNK[kBlkWords];
-  __shared__ BlockEdge edge;
-  __shared__ int item, changed[2], fb[3][4];
-  bfs_block_items<4, 1, kBlockRows, kBlockCols>(b, cur, direction, edge, sM, sV1, sV2, sNK, changed,
-                                                fb, item);
+// Boxes of the block list (up to 448 x 384 pixels, or 384 x 448, within the
+// frame's search window): one workgroup of 8 waves per label, one band of 56
+// rows per wave, the mask in registers.  The box is searched in the
+// orientation that needs fewer 64-bit words per band row (transposed on a
+// tie: its loads and stores need no cross-lane exchange); it fits the bands
+// whole, so no window or fallback search is needed here.
+typedef BandEdge<8, 6, 8> MidEdge;
+__global__ void __launch_bounds__(512) kb_bfs_mid(Batch b, int cur, int direction) {
+  __shared__ MidEdge edge;
+  __shared__ int item;
+  const int tid = threadIdx.x;
+  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
+  while (true) {
+    __syncthreads();
+    if (tid == 0) item = atomicAdd(b.work_n + 3, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(item);
+    if (it >= n_items) break;
+    const int wv = __builtin_amdgcn_readfirstlane(b.work[b.work_cap + it]);
+    const int f = wv >> 24, k = wv & 0xffffff;
+    const FrameDesc& F = b.fr[f];
+    const int* seg = F.seg[cur];
+    const int W = F.W;
+    const int* bb = F.st.bbox + 4 * k;
+    const int bx0 = __builtin_amdgcn_readfirstlane(bb[0]);
+    const int by0 = __builtin_amdgcn_readfirstlane(bb[2]);
+    const int bw = __builtin_amdgcn_readfirstlane(bb[1]) - bx0 + 1;
+    const int bh = __builtin_amdgcn_readfirstlane(bb[3]) - by0 + 1;
+    const int s1 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, direction));
+    const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
+    const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
+    if (s1 < 0) continue;  // no pixel has d1 > d2: the label keeps all its pixels
+    const bool fit_n = bh <= kMidRows && bw <= kMidCols;
+    const bool fit_t = bw <= kMidRows && bh <= kMidCols;
+    const int wd_n = (bw + 63) >> 6, wd_t = (bh + 63) >> 6;
+    const bool tr = !fit_n || (fit_t && wd_t <= wd_n);
+    const int wd = tr ? wd_t : wd_n;
+#define STSP_MID(WD, T)                                                                          \
+  (void)bfs_bands<8, 1, WD, false, MidEdge, T>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, F.halves, \
+                                                edge, 0, nullptr)
+    if (tr) {
+      switch (wd) {
+        case 1: STSP_MID(1, true); break;
+        case 2: STSP_MID(2, true); break;
+        case 3: STSP_MID(3, true); break;
+        case 4: STSP_MID(4, true); break;
+        case 5: STSP_MID(5, true); break;
+        default: STSP_MID(6, true); break;
+      }
+    } else {
+      switch (wd) {
+        case 1: STSP_MID(1, false); break;
+        case 2: STSP_MID(2, false); break;
+        case 3: STSP_MID(3, false); break;
+        case 4: STSP_MID(4, false); break;
+        case 5: STSP_MID(5, false); break;
+        default: STSP_MID(6, false); break;
+      }
+    }
+#undef STSP_MID
+    if (tid == 0) atomicAdd(F.search_count + kSearchBand, 1ull);
+  }
 }
 
 // One LDS pool: the band mask (16 bands x 8 words x 64 lanes) and the band
@@ -1030,7 +1113,7 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
     (void)hipStreamWaitEvent(side.st2, side.fork, 0);
   }
   kb_bfs_big<<<64, 512, 0, side.st ? side.st2 : s>>>(b, cur, direction);
-  kb_bfs_block<<<256, 256, 0, side.st ? side.st : s>>>(b, cur, direction);
+  kb_bfs_mid<<<256, 512, 0, side.st ? side.st : s>>>(b, cur, direction);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
   if (side.st) {
     (void)hipEventRecord(side.join, side.st);

@@ -26,6 +26,10 @@ CONFIG2 = dict(n=32, seed=0, cfg=dict(niters=500, sp_size=20, potts=10.0))
 CONFIG3_SAMPLE = [dict(T=20, H=360, W=640, nregions=120, vmax=4.0, seed=3000, check=4),
                   dict(T=20, H=1080, W=1920, nregions=120, vmax=4.0, seed=3003, check=4)]
 CONFIG3_CFG = dict(niters=100, sp_size=25, potts=0.5, prop_niters=100)
+# the 1080x1920 serving stream over 10 frames with config 1's iterations: from
+# its fifth frame on some superpixels grow past 128 x 128 pixel boxes, the
+# workgroup split searches (all 10 frames are checked against the oracle)
+CONFIG3_LONG = dict(T=10, H=1080, W=1920, nregions=120, vmax=4.0, seed=3003)
 
 
 def digest(*arrays) -> str:
@@ -72,3 +76,9 @@ def config3_sample_inputs(i, T=None):
     c = CONFIG3_SAMPLE[i]
     return synth.voronoi_video(T or c["T"], c["H"], c["W"], nregions=c["nregions"],
                                vmax=c["vmax"], seed=c["seed"])
+
+
+def config3_long_inputs():
+    c = CONFIG3_LONG
+    return synth.voronoi_video(c["T"], c["H"], c["W"], nregions=c["nregions"], vmax=c["vmax"],
+                               seed=c["seed"])

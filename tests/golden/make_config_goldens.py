@@ -4,7 +4,10 @@ tests/config_workloads.py, plus digests of their inputs (so a GPU test can
 tell whether it rebuilt the same inputs).  The oracle is the C restatement
 in oracle/; this script only runs it.  Usage:
 
-    python tests/golden/make_config_goldens.py [--jobs 8]
+    python tests/golden/make_config_goldens.py [--jobs 8] [--only NAME ...]
+
+With --only, the named entries are recomputed and the others kept from the
+existing file.
 """
 from __future__ import annotations
 
@@ -57,19 +60,41 @@ def config3_job(i):
             "frames": video_goldens(vid, flows, wl.CONFIG3_CFG, c["check"])}
 
 
+def config3_long_job():
+    vid, flows = wl.config3_long_inputs()
+    c = wl.CONFIG3_LONG
+    return {"inputs": wl.digest(vid, flows),
+            "frames": video_goldens(vid, flows, wl.CONFIG3_CFG, c["T"])}
+
+
+NAMES = ("config1", "config2", "config3_sample", "config3_long")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--only", nargs="*", choices=NAMES, default=None)
     args = ap.parse_args()
     oracle.build()
+    path = os.path.join(HERE, "config_goldens.json")
+    want = set(args.only or NAMES)
+    out = {}
+    if args.only and os.path.exists(path):
+        out = {k: v for k, v in json.load(open(path)).items() if k not in want}
     t0 = time.time()
     with cf.ProcessPoolExecutor(args.jobs) as ex:
-        f1 = ex.submit(config1_job)
-        f3 = [ex.submit(config3_job, i) for i in range(len(wl.CONFIG3_SAMPLE))]
-        f2 = [ex.submit(config2_job, i) for i in range(wl.CONFIG2["n"])]
-        out = {"config1": f1.result(), "config2": [f.result() for f in f2],
-               "config3_sample": [f.result() for f in f3]}
-    path = os.path.join(HERE, "config_goldens.json")
+        fut = {}
+        if "config1" in want:
+            fut["config1"] = ex.submit(config1_job)
+        if "config3_long" in want:
+            fut["config3_long"] = ex.submit(config3_long_job)
+        if "config3_sample" in want:
+            fut["config3_sample"] = [ex.submit(config3_job, i)
+                                     for i in range(len(wl.CONFIG3_SAMPLE))]
+        if "config2" in want:
+            fut["config2"] = [ex.submit(config2_job, i) for i in range(wl.CONFIG2["n"])]
+        for name, f in fut.items():
+            out[name] = [g.result() for g in f] if isinstance(f, list) else f.result()
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(f"wrote {path} in {time.time() - t0:.0f} s")

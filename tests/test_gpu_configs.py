@@ -70,6 +70,32 @@ def test_config1_480x854_100_iterations():
         assert wl.params_digest(host_params(p)) == e["params"], f"frame {t} params"
 
 
+def test_config3_1080p_stream_10_frames():
+    """The 1080x1920 serving stream (config 1's motion and iterations) for 10
+    frames: from frame 5 on some superpixels have boxes beyond 128 x 128,
+    which the workgroup split searches handle; every frame's labels and
+    parameters against the oracle's digests."""
+    c = wl.CONFIG3_LONG
+    vid, flows = wl.config3_long_inputs()
+    exp = expected_frames(GOLD["config3_long"], vid, flows, wl.CONFIG3_CFG, c["T"])
+    cfg = core.make_config(**wl.CONFIG3_CFG)
+    ctx = core.Context(c["H"], c["W"], cfg)
+    spix, params = core.bass_frame(cuda(vid[0]), cfg, ctx=ctx)
+    got = [(spix, params)]
+    for t in range(1, c["T"]):
+        spix, params, _ = core.prop_frame(cuda(vid[t]), cuda(flows[t - 1]), spix, params, cfg,
+                                          ctx=ctx)
+        got.append((spix, params))
+    torch.cuda.synchronize()
+    counts = ctx.split_search_counts()
+    ctx.close()
+    print("split searches", counts)
+    for t, ((s, p), e) in enumerate(zip(got, exp)):
+        assert wl.digest(s.cpu().numpy()) == e["labels"], f"frame {t} labels"
+        assert wl.params_digest(host_params(p)) == e["params"], f"frame {t} params"
+    assert counts["band"] > 0
+
+
 def test_config2_32_images_one_batch_graph_replayed():
     """Config 2: 32 images of sizes 240..720 x 320..1280 refined in shared
     launches, the refinement captured once as a hipGraph and replayed on a

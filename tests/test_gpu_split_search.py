@@ -60,7 +60,8 @@ def run_video(orc, T, H, W, nreg, sp, niters, seed, search, churn=0.0):
     return counts
 
 
-@pytest.mark.parametrize("search", [(1, 16, 64), (1, 8, 32), (16, 24, 64), (128, 448, 384)])
+@pytest.mark.parametrize("search", [(1, 16, 64), (1, 8, 32), (16, 24, 64), (128, 448, 384),
+                                    (16, 448, 384)])
 def test_split_search_paths_prop(orc, search):
     """Propagated video (relabelling can leave a label in several pieces, so
     windows that hold the seed's component occur beside windows it leaves)."""
@@ -78,3 +79,11 @@ def test_split_search_paths_churn(orc):
     counts = run_video(orc, 5, 90, 150, 30, 8, 16, seed=5, search=(1, 12, 64), churn=0.1)
     print("churn", counts)
     assert counts["window"] + counts["whole"] > 0
+
+
+def test_split_search_large_boxes(orc):
+    """Few, large superpixels: boxes of a few hundred pixels, searched whole by
+    the 8-wave kernel in both orientations (1 to 6 words per band row)."""
+    counts = run_video(orc, 4, 256, 480, 6, 90, 20, seed=8, search=(16, 448, 384))
+    print("large", counts)
+    assert counts["band"] > 0
