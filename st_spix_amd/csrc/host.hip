@@ -151,6 +151,7 @@ struct stsp_batch {
   int* d_pro_off = nullptr;
   int* d_fillcnt = nullptr;
   int* d_negs = nullptr;
+  int* d_hlen = nullptr;  // hole-list lengths per frame
   ProFrame* h_pro = nullptr;
   int* h_pro_off = nullptr;
   int* h_fillcnt = nullptr;
@@ -390,6 +391,7 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
   HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf * kNegSub));
+  HIP_TRY(hipMalloc((void**)&bh->d_hlen, sizeof(int) * maxf));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro, sizeof(ProFrame) * maxf, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro_off, sizeof(int) * kProKinds * (kProMax + 1),
                         hipHostMallocDefault));
@@ -479,6 +481,7 @@ static void batch_free(stsp_batch* bh) {
   if (bh->d_pro_off) (void)hipFree(bh->d_pro_off);
   if (bh->d_fillcnt) (void)hipFree(bh->d_fillcnt);
   if (bh->d_negs) (void)hipFree(bh->d_negs);
+  if (bh->d_hlen) (void)hipFree(bh->d_hlen);
   if (bh->h_pro) (void)hipHostFree(bh->h_pro);
   if (bh->h_pro_off) (void)hipHostFree(bh->h_pro_off);
   if (bh->h_fillcnt) (void)hipHostFree(bh->h_fillcnt);
@@ -488,7 +491,7 @@ static void batch_free(stsp_batch* bh) {
   bh->d_live = bh->h_live = nullptr;
   bh->d_pro = nullptr;
   bh->d_pro_off = nullptr;
-  bh->d_fillcnt = bh->d_negs = nullptr;
+  bh->d_fillcnt = bh->d_negs = bh->d_hlen = nullptr;
   bh->h_pro = nullptr;
   bh->h_pro_off = nullptr;
   bh->h_fillcnt = bh->h_negs = nullptr;
@@ -980,6 +983,7 @@ static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStre
   int qtotal[kProKinds];
   for (int k = 0; k < kProKinds; k++) qtotal[k] = total[k];
   int chunk = 4;  // most frames finish within a few rounds
+  pro_hole_list(p, total, s);  // every frame's holes at the start of the fill
   for (int base = 0; base < kFillMax; base += chunk, chunk = kFillChunk) {
     const int rounds = kFillMax - base < chunk ? kFillMax - base : chunk;
     pro_fill_rounds(q, qtotal, base, rounds, s);
@@ -1096,6 +1100,7 @@ static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* co
   p.nall = n;
   p.fillcnt = bh->d_fillcnt;
   p.negs = bh->d_negs;
+  p.hlen = bh->d_hlen;
   pro_shift(p, total, s);
   int rc = fill_loop_pro(bh, p, total, s);
   if (rc) return rc;

@@ -56,6 +56,10 @@ int pro_blocks(int kind, int W, int H, int np, int nbuf) {
     case kProLab: return div_up(np, kBlock);
     case kProBuf: return div_up(nbuf, kBlock);
     case kProTile: return div_up(W, 32) * div_up(H, 32);
+    case kProHole: {
+      const int nb = div_up((long long)W * H, kBlock);
+      return nb < kHoleBlocks ? nb : kHoleBlocks;
+    }
     default: return div_up((long long)((W + 1) / 2) * ((H + 1) / 2), kBlock);  // kProHalf
   }
 }
@@ -184,9 +188,51 @@ void pro_shift(const Pro& p, const int* total, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // hole filling (fill_missing.cu:35-130, 288-441, 617-647).  Round r counts
 // the holes and marks the border (a valid 4-neighbour) at its start, then
-// fills border holes in four checkerboard phases.  A frame whose count was
+// fills border holes in four checkerboard phases.  A fill never makes a new
+// hole, so the pixels that are holes when a fill starts (listed once, in the
+// frame's `keys` buffer, free after the shift) hold every hole of its
+// rounds: the rounds visit the list, not the frame.  A frame whose count was
 // zero in the round before is finished: its blocks return at once.
 // ---------------------------------------------------------------------------
+// adds the block's sum of v to *dst with one atomic (all threads call it)
+__device__ __forceinline__ void block_sum_add(int v, int* dst) {
+  __shared__ int ws[kBlock / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int w = 0; w < kBlock / 64; w++) n += ws[w];
+    if (n) atomicAdd(dst, n);
+  }
+}
+
+__global__ void kp_hole_clear(Pro p) {
+  if (threadIdx.x < p.nall) p.hlen[threadIdx.x] = 0;
+}
+
+// the holes of every frame (labels < 0), appended with one atomic per wave
+__global__ void kp_hole_list(Pro p) {
+  PRO_PIX
+  const bool hole = t < F.npix && F.seg[t] < 0;
+  const unsigned long long b = __ballot(hole);
+  if (!b) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == __ffsll((long long)b) - 1) base = atomicAdd(p.hlen + f, __popcll(b));
+  base = __shfl(base, __ffsll((long long)b) - 1);
+  if (hole) {
+    int* list = reinterpret_cast<int*>(F.keys);
+    list[base + __popcll(b & ((1ull << lane) - 1ull))] = t;
+  }
+}
+
+void pro_hole_list(const Pro& p, const int* total, hipStream_t s) {
+  kp_hole_clear<<<1, kProMax, 0, s>>>(p);
+  kp_hole_list<<<total[kProPix], kBlock, 0, s>>>(p);
+}
+
 // The filling rounds are launched over the frames that still have holes
 // (p.fmap lists them; the counts keep the table's frame index).
 __global__ void kp_fill_zero(Pro p, int base, int rounds) {
@@ -194,69 +240,77 @@ __global__ void kp_fill_zero(Pro p, int base, int rounds) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nall + i] = 0;
 }
 
+// the blocks of frame fl of a hole-list launch: first block and count
+#define PRO_HOLE                                                           \
+  const int fl = pro_frame(p, kProHole, blockIdx.x);                       \
+  const int f = p.fmap ? p.fmap[fl] : fl;                                  \
+  const ProFrame& F = p.fr[f];                                             \
+  const int* off_h = p.off + kProHole * (kProMax + 1);                     \
+  const int hb = blockIdx.x - off_h[fl], nhb = off_h[fl + 1] - off_h[fl];  \
+  const int* list = reinterpret_cast<const int*>(F.keys);                  \
+  const int nl = p.hlen[f];
+
 __global__ void kp_fill_border(Pro p, int r) {
-  const int fl = pro_frame(p, kProPix, blockIdx.x);
-  const int f = p.fmap ? p.fmap[fl] : fl;
-  const ProFrame& F = p.fr[f];
-  const int t = pro_local(p, kProPix, fl, blockIdx.x) * kBlock + threadIdx.x;
+  PRO_HOLE
   if (r > 0 && p.fillcnt[(r - 1) * p.nall + f] == 0) return;  // block-uniform
-  bool neg = false;
-  if (t < F.npix) {
-    const int W = F.W, H = F.H;
-    const int* seg = F.seg;
-    int x = t % W, y = t / W;
-    int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
-    int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
+  const int W = F.W, H = F.H;
+  const int* seg = F.seg;
+  int neg = 0;
+  for (int i = hb * kBlock + threadIdx.x; i < nl; i += nhb * kBlock) {
+    const int t = list[i];
+    if (seg[t] >= 0) continue;  // filled in an earlier round
+    const int x = t % W, y = t / W;
+    const int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
+    const int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
     F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
-    neg = seg[t] < 0;
+    neg++;
   }
-  block_count_add(neg, p.fillcnt + r * p.nall + f);
+  block_sum_add(neg, p.fillcnt + r * p.nall + f);
 }
 
 __global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {
-  const int fl = pro_frame(p, kProHalf, blockIdx.x);
-  const int f = p.fmap ? p.fmap[fl] : fl;
+  PRO_HOLE
   if (p.fillcnt[r * p.nall + f] == 0) return;
-  const ProFrame& F = p.fr[f];
   const int W = F.W, H = F.H;
-  int hw = (W - xm + 1) / 2, hh = (H - ym + 1) / 2;
-  int i = pro_local(p, kProHalf, fl, blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= hw * hh) return;
   int* seg = F.seg;
-  int x = 2 * (i % hw) + xm, y = 2 * (i / hw) + ym;
-  int idx = y * W + x;
-  if (!F.border[idx] || seg[idx] != -1) return;
-  int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
-  int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
-  const int cand[4] = {N, S, Wl, E};
-  float best = -999999.0f;
-  int lab = -1;
-  for (int j = 0; j < 4; j++) {
-    int L = cand[j];
-    if (L < 0) continue;
-    bool dup = false;
-    for (int q = 0; q < j; q++) dup |= cand[q] == L;
-    if (dup) continue;
-    // isotropic_space, fill_missing.cu:140-153
-    float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
-    float dy =
-        (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
-    float sim = -dx * dx - dy * dy;
-    if (sim > best) {
-      best = sim;
-      lab = L;
+  for (int i = hb * kBlock + threadIdx.x; i < nl; i += nhb * kBlock) {
+    const int idx = list[i];
+    const int x = idx % W, y = idx / W;
+    if ((x & 1) != xm || (y & 1) != ym) continue;
+    if (seg[idx] != -1 || !F.border[idx]) continue;
+    const int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
+    const int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
+    const int cand[4] = {N, S, Wl, E};
+    float best = -999999.0f;
+    int lab = -1;
+    for (int j = 0; j < 4; j++) {
+      int L = cand[j];
+      if (L < 0) continue;
+      bool dup = false;
+      for (int q = 0; q < j; q++) dup |= cand[q] == L;
+      if (dup) continue;
+      // isotropic_space, fill_missing.cu:140-153
+      float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
+      float dy =
+          (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
+      float sim = -dx * dx - dy * dy;
+      if (sim > best) {
+        best = sim;
+        lab = L;
+      }
     }
+    seg[idx] = lab;
   }
-  seg[idx] = lab;
 }
+#undef PRO_HOLE
 
 void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s) {
   kp_fill_zero<<<1, 256, 0, s>>>(p, base, rounds);
   for (int r = base; r < base + rounds; r++) {
-    kp_fill_border<<<total[kProPix], kBlock, 0, s>>>(p, r);
+    kp_fill_border<<<total[kProHole], kBlock, 0, s>>>(p, r);
     for (int xm = 0; xm < 2; xm++)
       for (int ym = 0; ym < 2; ym++)
-        kp_fill_phase<<<total[kProHalf], kBlock, 0, s>>>(p, r, xm, ym);
+        kp_fill_phase<<<total[kProHole], kBlock, 0, s>>>(p, r, xm, ym);
   }
 }
 

@@ -72,7 +72,9 @@ struct ProFrame {
   uint8_t* border;
   unsigned long long* best;
 };
-enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProKinds };
+enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProHole, kProKinds };
+// blocks per frame of the hole-list kernels (grid-stride over the list)
+constexpr int kHoleBlocks = 64;
 // partial hole counters per frame: the counting blocks of a frame spread their
 // atomics over kNegSub addresses instead of one
 constexpr int kNegSub = 64;
@@ -86,11 +88,14 @@ struct Pro {
   int* fillcnt;        // [kFillMax][nall]: holes at the start of each round
   int* negs;           // [nall * kNegSub]: holes of the shifted map, kNegSub partial
                        // counters per frame (summed by the host)
+  int* hlen;           // [nall]: length of each frame's hole list (pixel indices of
+                       // the holes when a fill starts, kept in the frame's `keys`)
 };
 // host side: block counts per kind for a frame
 int pro_blocks(int kind, int W, int H, int np, int nbuf);
 // the whole prologue except the hole-filling rounds
 void pro_shift(const Pro& p, const int* total, hipStream_t s);
+void pro_hole_list(const Pro& p, const int* total, hipStream_t s);
 void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s);
 void pro_disconnected(const Pro& p, const int* total, hipStream_t s);
 void pro_finish(const Pro& p, const int* total, hipStream_t s);
