@@ -55,6 +55,11 @@ struct FrameDesc {
 // component; the whole box after the component left the window.
 enum { kSearchWave = 0, kSearchBand, kSearchWindow, kSearchWhole, kSearchPaths };
 constexpr int kWaveBoxMax = 128;
+// per-pixel launches of the refinement loop: 4 pixels per thread (the block's
+// frame lookup is paid once per 1024 pixels), pixel j * kBlock + thread of
+// the block's 1024
+constexpr int kPixPerThread = 4;
+constexpr int kPixBlock = kBlock * kPixPerThread;
 // label slots of a frame: labels fill 24-bit fields (work list entries
 // frame << 24 | label, the sweep's packed LDS words)
 constexpr int kLabelLimit = 1 << 24;
@@ -73,7 +78,7 @@ constexpr SweepShape kSweepShape[kSweepShapes] = {
 
 // prefix offsets of one batch (device), each nf+1 long
 struct BatchOff {
-  int* pix;      // 256-pixel blocks
+  int* pix;      // kPixBlock-pixel blocks
   int* lab;      // 256-label blocks over cap
   int* sm;       // 256-slot blocks over 2*cap+2
   int* tile[kSweepShapes];  // sweep tiles for each launch shape

@@ -391,7 +391,7 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
   HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf * kNegSub));
-  HIP_TRY(hipMalloc((void**)&bh->d_hlen, sizeof(int) * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_hlen, sizeof(int) * maxf * kHoleSub));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro, sizeof(ProFrame) * maxf, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro_off, sizeof(int) * kProKinds * (kProMax + 1),
                         hipHostMallocDefault));
@@ -661,7 +661,7 @@ static int batch_build(stsp_batch* bh, stsp_ctx** ctxs, int n, hipStream_t s, in
   for (int i = 0; i < n; i++) {
     stsp_ctx* c = ctxs[i];
     hfr[i] = make_desc(c);
-    o_pix[i + 1] = o_pix[i] + div_up(c->npix, kBlock);
+    o_pix[i + 1] = o_pix[i] + div_up(c->npix, kPixBlock);
     o_lab[i + 1] = o_lab[i] + div_up(c->cap, kBlock);
     o_sm[i + 1] = o_sm[i] + div_up(2 * c->cap + 2, kBlock);
     for (int sh = 0; sh < kSweepShapes; sh++)

@@ -361,33 +361,60 @@ void b_count_living(const Batch& b, int slot, hipStream_t s) {
 // per-frame helpers
 // ---------------------------------------------------------------------------
 // one workgroup per frame: exclusive scan of n = nbuf entries; which selects
-// (split accept -> sm.rank) or (relabel want_new -> rank); total -> F.total
+// (split accept -> sm.rank) or (relabel want_new -> rank); total -> F.total.
+// Tiles of 4096 entries (4 consecutive per thread), the next tile's loads in
+// flight while one is scanned: a thread's 4 sums, a wave scan of the thread
+// totals, then the 16 wave totals through LDS.
 __global__ void __launch_bounds__(1024) kb_scan(Batch b, int which) {
   const FrameDesc& F = b.fr[blockIdx.x];
-  const int* in = which == 0 ? F.sm.accept : F.want_new;
-  int* out = which == 0 ? F.sm.rank : F.rank;
+  const int* __restrict__ in = which == 0 ? F.sm.accept : F.want_new;
+  int* __restrict__ out = which == 0 ? F.sm.rank : F.rank;
   const int n = F.nbuf;
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int beg = tid * chunk, end = min(n, beg + chunk);
-  int sum = 0;
-  for (int i = beg; i < end; i++) sum += in[i];
-  part[tid] = sum;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    int v = tid >= d ? part[tid - d] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  constexpr int kTile = 4096;
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int v[4], nx[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int i = 4 * tid + j;
+    v[j] = i < n ? in[i] : 0;
   }
-  int run = part[tid] - sum;
-  for (int i = beg; i < end; i++) {
-    int v = in[i];
-    out[i] = run;
-    run += v;
+  int carry = 0;
+  for (int base = 0; base < n; base += kTile) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = base + kTile + 4 * tid + j;
+      nx[j] = i < n ? in[i] : 0;
+    }
+    const int s0 = v[0], s1 = s0 + v[1], s2 = s1 + v[2], s3 = s2 + v[3];
+    int x = s3;  // inclusive scan of the thread totals over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int wpre = 0, tile = 0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+      const int t = wsum[w];
+      wpre += w < wv ? t : 0;
+      tile += t;
+    }
+    const int ex = carry + wpre + x - s3;
+    const int pre[4] = {0, s0, s1, s2};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = base + 4 * tid + j;
+      if (i < n) out[i] = ex + pre[j];
+    }
+    carry += tile;
+    __syncthreads();  // wsum is rewritten by the next tile
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = nx[j];
   }
-  if (tid == 1023) *F.total = part[1023];
+  if (tid == 0) *F.total = carry;
 }
 
 __global__ void kb_reset_work(Batch b) {
@@ -435,25 +462,36 @@ __global__ void kb_sm_clear(Batch b) {
 // 4-neighbour of another label), the border test done inline.
 __global__ void kb_merge_cand(Batch b, int cur, int direction) {
   PIX_PROLOGUE
-  if (t >= F.npix) return;
   const int W = F.W, H = F.H;
   const int* seg = F.seg[cur];
-  const int prop_mode = F.prop_mode;
-  int x = t % W, y = t / W;
-  int C = seg[t], Wn = -1;
-  if (y >= 1 && x >= 1 && y < H - 1 && x < W - 1 && C == seg[t - W] && C == seg[t + W] &&
-      C == seg[t + 1] && C == seg[t - 1])
-    return;
-  if (direction == 1) {
-    bool ok = prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
-    if (ok) Wn = seg[t + W];
-  } else {
-    bool ok = prop_mode ? (x > 1 && x < W - 2) : (x > 0 && x < W);
-    if (ok) Wn = seg[t - 1];
-  }
-  if (Wn >= 0 && C != Wn) {
-    if (prop_mode && (F.P.prop[C] == F.P.prop[Wn])) return;
-    atomicMax(F.sm.pairs + 2 * C + 1, Wn);
+  // (x, y) of pixel t0, then stepped by kBlock pixels (no division per pixel)
+  int y = t0 / W, x = t0 - y * W;
+  for (int j = 0; j < kPixPerThread; j++) {
+    if (j > 0) {
+      x += kBlock;
+      while (x >= W) {
+        x -= W;
+        y++;
+      }
+    }
+    const int t = t0 + j * kBlock;
+    if (t >= F.npix) break;
+    const int C = seg[t];
+    const int N = y > 0 ? seg[t - W] : C, S = y < H - 1 ? seg[t + W] : C;
+    const int Wl = x > 0 ? seg[t - 1] : C, E = x < W - 1 ? seg[t + 1] : C;
+    const bool inner = y >= 1 && x >= 1 && y < H - 1 && x < W - 1 && C == N && C == S &&
+                       C == E && C == Wl;
+    if (inner) continue;
+    int Wn = -1;
+    if (direction == 1) {
+      const bool ok = F.prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
+      if (ok) Wn = S;
+    } else {
+      const bool ok = F.prop_mode ? (x > 1 && x < W - 2) : (x > 0 && x < W);
+      if (ok) Wn = Wl;
+    }
+    if (Wn >= 0 && C != Wn && !(F.prop_mode && F.P.prop[C] == F.P.prop[Wn]))
+      atomicMax(F.sm.pairs + 2 * C + 1, Wn);
   }
 }
 
@@ -550,10 +588,17 @@ __global__ void kb_merge_remove(Batch b) {
 
 __global__ void kb_merge_apply(Batch b, int cur) {
   PIX_PROLOGUE
-  if (t >= F.npix) return;
   int* seg = F.seg[cur];
-  int k = seg[t];
-  if (k >= 0 && k < F.nbuf && F.sm.remove[k]) seg[t] = F.sm.pairs[2 * k + 1];
+  int k[kPixPerThread];
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    const int t = t0 + j * kBlock;
+    k[j] = t < F.npix ? seg[t] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++)
+    if (k[j] >= 0 && k[j] < F.nbuf && F.sm.remove[k[j]])
+      seg[t0 + j * kBlock] = F.sm.pairs[2 * k[j] + 1];
 }
 
 // Label statistics after a merge without a recount: the merge's own pass
@@ -631,9 +676,15 @@ __global__ void kb_relabel_prep(Batch b) {
 
 __global__ void kb_mark_alive(Batch b, int cur) {
   PIX_PROLOGUE
-  if (t >= F.npix) return;
-  int k = F.seg[cur][t];
-  if (k >= 0 && k < F.nbuf) F.alive[k] = 1;
+  int k[kPixPerThread];
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    const int t = t0 + j * kBlock;
+    k[j] = t < F.npix ? F.seg[cur][t] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++)
+    if (k[j] >= 0 && k[j] < F.nbuf) F.alive[k[j]] = 1;
 }
 
 __global__ void kb_relabel_lists(Batch b) {
@@ -751,12 +802,18 @@ __global__ void kb_relabel_valid(Batch b) {
 
 __global__ void kb_relabel_apply(Batch b, int cur) {
   PIX_PROLOGUE
-  if (t >= F.npix) return;
   int* seg = F.seg[cur];
-  int L = seg[t];
-  if (L >= 0 && L < F.nbuf) {
-    int tg = F.target[L];
-    if (tg >= 0) seg[t] = tg;
+  int L[kPixPerThread];
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    const int t = t0 + j * kBlock;
+    L[j] = t < F.npix ? seg[t] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    if (L[j] < 0 || L[j] >= F.nbuf) continue;
+    const int tg = F.target[L[j]];
+    if (tg >= 0) seg[t0 + j * kBlock] = tg;
   }
 }
 

@@ -58,7 +58,7 @@ int pro_blocks(int kind, int W, int H, int np, int nbuf) {
     case kProTile: return div_up(W, 32) * div_up(H, 32);
     case kProHole: {
       const int nb = div_up((long long)W * H, kBlock);
-      return nb < kHoleBlocks ? nb : kHoleBlocks;
+      return nb < kHoleSub ? nb : kHoleSub;
     }
     default: return div_up((long long)((W + 1) / 2) * ((H + 1) / 2), kBlock);  // kProHalf
   }
@@ -209,7 +209,13 @@ __device__ __forceinline__ void block_sum_add(int v, int* dst) {
 }
 
 __global__ void kp_hole_clear(Pro p) {
-  if (threadIdx.x < p.nall) p.hlen[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < p.nall * kHoleSub; i += blockDim.x) p.hlen[i] = 0;
+}
+
+// entries of one hole sub-list of a frame of npix pixels: the pixel blocks
+// that append to it, 256 pixels each
+__device__ __forceinline__ int hole_sub_cap(int npix) {
+  return ((npix + kBlock - 1) / kBlock + kHoleSub - 1) / kHoleSub * kBlock;
 }
 
 // the holes of every frame (labels < 0), appended with one atomic per wave
@@ -218,18 +224,20 @@ __global__ void kp_hole_list(Pro p) {
   const bool hole = t < F.npix && F.seg[t] < 0;
   const unsigned long long b = __ballot(hole);
   if (!b) return;
+  const int sub = pro_local(p, kProPix, f, blockIdx.x) % kHoleSub;
   const int lane = threadIdx.x & 63;
+  const int first = __ffsll((long long)b) - 1;
   int base = 0;
-  if (lane == __ffsll((long long)b) - 1) base = atomicAdd(p.hlen + f, __popcll(b));
-  base = __shfl(base, __ffsll((long long)b) - 1);
+  if (lane == first) base = atomicAdd(p.hlen + f * kHoleSub + sub, __popcll(b));
+  base = __shfl(base, first);
   if (hole) {
-    int* list = reinterpret_cast<int*>(F.keys);
+    int* list = reinterpret_cast<int*>(F.keys) + (size_t)sub * hole_sub_cap(F.npix);
     list[base + __popcll(b & ((1ull << lane) - 1ull))] = t;
   }
 }
 
 void pro_hole_list(const Pro& p, const int* total, hipStream_t s) {
-  kp_hole_clear<<<1, kProMax, 0, s>>>(p);
+  kp_hole_clear<<<1, 1024, 0, s>>>(p);
   kp_hole_list<<<total[kProPix], kBlock, 0, s>>>(p);
 }
 
@@ -240,7 +248,8 @@ __global__ void kp_fill_zero(Pro p, int base, int rounds) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nall + i] = 0;
 }
 
-// the blocks of frame fl of a hole-list launch: first block and count
+// the blocks of frame fl of a hole-list launch: block hb of nhb visits the
+// sub-lists hb, hb + nhb, ... (entries i of sub-list sub: HOLE_AT(sub, i))
 #define PRO_HOLE                                                           \
   const int fl = pro_frame(p, kProHole, blockIdx.x);                       \
   const int f = p.fmap ? p.fmap[fl] : fl;                                  \
@@ -248,7 +257,8 @@ __global__ void kp_fill_zero(Pro p, int base, int rounds) {
   const int* off_h = p.off + kProHole * (kProMax + 1);                     \
   const int hb = blockIdx.x - off_h[fl], nhb = off_h[fl + 1] - off_h[fl];  \
   const int* list = reinterpret_cast<const int*>(F.keys);                  \
-  const int nl = p.hlen[f];
+  const int subcap = hole_sub_cap(F.npix);                                 \
+  const int* hl = p.hlen + f * kHoleSub;
 
 __global__ void kp_fill_border(Pro p, int r) {
   PRO_HOLE
@@ -256,14 +266,16 @@ __global__ void kp_fill_border(Pro p, int r) {
   const int W = F.W, H = F.H;
   const int* seg = F.seg;
   int neg = 0;
-  for (int i = hb * kBlock + threadIdx.x; i < nl; i += nhb * kBlock) {
-    const int t = list[i];
-    if (seg[t] >= 0) continue;  // filled in an earlier round
-    const int x = t % W, y = t / W;
-    const int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
-    const int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
-    F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
-    neg++;
+  for (int sub = hb; sub < kHoleSub; sub += nhb) {
+    for (int i = threadIdx.x; i < hl[sub]; i += kBlock) {
+      const int t = list[(size_t)sub * subcap + i];
+      if (seg[t] >= 0) continue;  // filled in an earlier round
+      const int x = t % W, y = t / W;
+      const int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
+      const int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
+      F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
+      neg++;
+    }
   }
   block_sum_add(neg, p.fillcnt + r * p.nall + f);
 }
@@ -273,33 +285,35 @@ __global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {
   if (p.fillcnt[r * p.nall + f] == 0) return;
   const int W = F.W, H = F.H;
   int* seg = F.seg;
-  for (int i = hb * kBlock + threadIdx.x; i < nl; i += nhb * kBlock) {
-    const int idx = list[i];
-    const int x = idx % W, y = idx / W;
-    if ((x & 1) != xm || (y & 1) != ym) continue;
-    if (seg[idx] != -1 || !F.border[idx]) continue;
-    const int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
-    const int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
-    const int cand[4] = {N, S, Wl, E};
-    float best = -999999.0f;
-    int lab = -1;
-    for (int j = 0; j < 4; j++) {
-      int L = cand[j];
-      if (L < 0) continue;
-      bool dup = false;
-      for (int q = 0; q < j; q++) dup |= cand[q] == L;
-      if (dup) continue;
-      // isotropic_space, fill_missing.cu:140-153
-      float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
-      float dy =
-          (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
-      float sim = -dx * dx - dy * dy;
-      if (sim > best) {
-        best = sim;
-        lab = L;
+  for (int sub = hb; sub < kHoleSub; sub += nhb) {
+    for (int i = threadIdx.x; i < hl[sub]; i += kBlock) {
+      const int idx = list[(size_t)sub * subcap + i];
+      const int x = idx % W, y = idx / W;
+      if ((x & 1) != xm || (y & 1) != ym) continue;
+      if (seg[idx] != -1 || !F.border[idx]) continue;
+      const int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
+      const int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
+      const int cand[4] = {N, S, Wl, E};
+      float best = -999999.0f;
+      int lab = -1;
+      for (int j = 0; j < 4; j++) {
+        int L = cand[j];
+        if (L < 0) continue;
+        bool dup = false;
+        for (int q = 0; q < j; q++) dup |= cand[q] == L;
+        if (dup) continue;
+        // isotropic_space, fill_missing.cu:140-153
+        float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
+        float dy =
+            (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
+        float sim = -dx * dx - dy * dy;
+        if (sim > best) {
+          best = sim;
+          lab = L;
+        }
       }
+      seg[idx] = lab;
     }
-    seg[idx] = lab;
   }
 }
 #undef PRO_HOLE

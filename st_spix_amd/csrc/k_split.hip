@@ -1040,19 +1040,27 @@ __global__ void kb_split_assign(Batch b) {
 
 __global__ void kb_split_apply(Batch b, int cur) {
   PIX_PROLOGUE
-  if (t >= F.npix) return;
   const int ms = *F.max_spix;
   if (ms > F.nbuf / 2) return;
   int* seg = F.seg[cur];
-  int k = seg[t];
-  if (k < 0 || k >= F.nbuf || !F.sm.accept[k]) return;
-  const int k2 = k + (ms + 1);
-  const int h = F.halves[t];
-  const bool take = F.prop_mode ? (F.sm.select[k] ? (h == k2) : (h == k)) : (h == k2);
-  if (take) {
-    int s = F.sm.pairs[2 * k];
-    seg[t] = s;
-    F.P.valid[s] = 1;
+  int k[kPixPerThread];
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    const int t = t0 + j * kBlock;
+    k[j] = t < F.npix ? seg[t] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++) {
+    if (k[j] < 0 || k[j] >= F.nbuf || !F.sm.accept[k[j]]) continue;
+    const int t = t0 + j * kBlock;
+    const int k2 = k[j] + (ms + 1);
+    const int h = F.halves[t];
+    const bool take = F.prop_mode ? (F.sm.select[k[j]] ? (h == k2) : (h == k[j])) : (h == k2);
+    if (take) {
+      const int sl = F.sm.pairs[2 * k[j]];
+      seg[t] = sl;
+      F.P.valid[sl] = 1;
+    }
   }
 }
 

@@ -30,10 +30,11 @@ __device__ __forceinline__ double nig_marg(const NigConsts& c, double n, const d
   return m;
 }
 
+// per-pixel kernels: pixels t0 + j * kBlock for j < kPixPerThread
 #define PIX_PROLOGUE                                         \
   const int f = frame_of(b.off.pix, b.nf, blockIdx.x);       \
   const FrameDesc& F = b.fr[f];                              \
-  const int t = (blockIdx.x - b.off.pix[f]) * kBlock + threadIdx.x;
+  const int t0 = (blockIdx.x - b.off.pix[f]) * kPixBlock + threadIdx.x;
 
 #define LAB_PROLOGUE                                         \
   const int f = frame_of(b.off.lab, b.nf, blockIdx.x);       \

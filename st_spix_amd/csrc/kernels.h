@@ -73,8 +73,10 @@ struct ProFrame {
   unsigned long long* best;
 };
 enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProHole, kProKinds };
-// blocks per frame of the hole-list kernels (grid-stride over the list)
-constexpr int kHoleBlocks = 64;
+// A frame's holes are kept in kHoleSub sub-lists (pixel block b appends to
+// sub-list b % kHoleSub, so the appends of a frame spread over kHoleSub
+// counters); the hole-list kernels have at most kHoleSub blocks per frame.
+constexpr int kHoleSub = 64;
 // partial hole counters per frame: the counting blocks of a frame spread their
 // atomics over kNegSub addresses instead of one
 constexpr int kNegSub = 64;
@@ -88,8 +90,9 @@ struct Pro {
   int* fillcnt;        // [kFillMax][nall]: holes at the start of each round
   int* negs;           // [nall * kNegSub]: holes of the shifted map, kNegSub partial
                        // counters per frame (summed by the host)
-  int* hlen;           // [nall]: length of each frame's hole list (pixel indices of
-                       // the holes when a fill starts, kept in the frame's `keys`)
+  int* hlen;           // [nall * kHoleSub]: lengths of each frame's hole sub-lists
+                       // (pixel indices of the holes when a fill starts, kept in
+                       // the frame's `keys`)
 };
 // host side: block counts per kind for a frame
 int pro_blocks(int kind, int W, int H, int np, int nbuf);
