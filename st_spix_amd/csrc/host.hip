@@ -101,7 +101,6 @@ struct KTimer {
 // small launches of different frames overlap.  All streams have the default
 // priority.  With lanes == 0 everything runs on the caller's stream.
 constexpr int kLanes = 4;
-constexpr int kFillChunk = 8;  // hole-filling rounds between two host checks
 struct Lanes {
   hipStream_t st[kLanes] = {};
   hipEvent_t fork = nullptr;
@@ -149,15 +148,13 @@ struct stsp_batch {
   // counts ([kFillMax][nf] on the device, one chunk of rounds on the host)
   ProFrame* d_pro = nullptr;
   int* d_pro_off = nullptr;
-  int* d_fillcnt = nullptr;
+  int* d_fillres = nullptr;  // per frame: filling rounds run, stall flag
   int* d_negs = nullptr;
   int* d_hlen = nullptr;  // hole-list lengths per frame
   ProFrame* h_pro = nullptr;
   int* h_pro_off = nullptr;
-  int* h_fillcnt = nullptr;
+  int* h_fillres = nullptr;
   int* h_negs = nullptr;
-  int* d_live = nullptr;  // frames still filling: offsets [kProKinds][kProMax + 1], then map
-  int* h_live = nullptr;
 };
 
 struct stsp_ctx {
@@ -389,19 +386,16 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
   HIP_TRY(hipHostMalloc((void**)&bh->h_off, off_bytes, hipHostMallocDefault));
   HIP_TRY(hipMalloc((void**)&bh->d_pro, sizeof(ProFrame) * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_pro_off, sizeof(int) * kProKinds * (kProMax + 1)));
-  HIP_TRY(hipMalloc((void**)&bh->d_fillcnt, sizeof(int) * kFillMax * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_fillres, sizeof(int) * 2 * maxf));
   HIP_TRY(hipMalloc((void**)&bh->d_negs, sizeof(int) * maxf * kNegSub));
   HIP_TRY(hipMalloc((void**)&bh->d_hlen, sizeof(int) * maxf * kHoleSub));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro, sizeof(ProFrame) * maxf, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_pro_off, sizeof(int) * kProKinds * (kProMax + 1),
                         hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void**)&bh->h_fillcnt, sizeof(int) * kFillChunk * maxf,
+  HIP_TRY(hipHostMalloc((void**)&bh->h_fillres, sizeof(int) * 2 * maxf,
                         hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf * kNegSub,
                         hipHostMallocDefault));
-  const size_t live = sizeof(int) * (kProKinds * (kProMax + 1) + kProMax);
-  HIP_TRY(hipMalloc((void**)&bh->d_live, live));
-  HIP_TRY(hipHostMalloc((void**)&bh->h_live, live, hipHostMallocDefault));
   return STSP_OK;
 }
 
@@ -479,22 +473,19 @@ static void batch_free(stsp_batch* bh) {
   if (bh->h_off) (void)hipHostFree(bh->h_off);
   if (bh->d_pro) (void)hipFree(bh->d_pro);
   if (bh->d_pro_off) (void)hipFree(bh->d_pro_off);
-  if (bh->d_fillcnt) (void)hipFree(bh->d_fillcnt);
+  if (bh->d_fillres) (void)hipFree(bh->d_fillres);
   if (bh->d_negs) (void)hipFree(bh->d_negs);
   if (bh->d_hlen) (void)hipFree(bh->d_hlen);
   if (bh->h_pro) (void)hipHostFree(bh->h_pro);
   if (bh->h_pro_off) (void)hipHostFree(bh->h_pro_off);
-  if (bh->h_fillcnt) (void)hipHostFree(bh->h_fillcnt);
+  if (bh->h_fillres) (void)hipHostFree(bh->h_fillres);
   if (bh->h_negs) (void)hipHostFree(bh->h_negs);
-  if (bh->d_live) (void)hipFree(bh->d_live);
-  if (bh->h_live) (void)hipHostFree(bh->h_live);
-  bh->d_live = bh->h_live = nullptr;
   bh->d_pro = nullptr;
   bh->d_pro_off = nullptr;
-  bh->d_fillcnt = bh->d_negs = bh->d_hlen = nullptr;
+  bh->d_fillres = bh->d_negs = bh->d_hlen = nullptr;
   bh->h_pro = nullptr;
   bh->h_pro_off = nullptr;
-  bh->h_fillcnt = bh->h_negs = nullptr;
+  bh->h_fillres = bh->h_negs = nullptr;
   bh->d_fr = nullptr;
   bh->d_off = nullptr;
   bh->d_work = nullptr;
@@ -964,73 +955,19 @@ static int bass_prologue_n(stsp_ctx** cs, const float* const* imgs, int n, Lanes
   return lanes_join(L, s);
 }
 
-// Hole filling (fill_missing.cu:35-130) of frames cs[0..n) together.  The
-// reference's loop is: count the holes; stop at zero; fail if the count did
-// not shrink; else one round of the four fill phases.  Here each frame runs a
-// chunk of such rounds without waiting (the count of round r goes to
-// fillcnt[r]), and one wait reads all the counts.  Rounds after the count
-// reached zero change nothing (no hole is left to fill), and a stall is
-// found in the recorded counts, so the outcome is the reference's.
-// Hole-filling rounds of every frame until none has holes left
-// (fill_missing.cu:63-115): rounds run in chunks, the hole counts of a chunk
-// are read back once, and a count that stops shrinking is an error as in the
-// reference (fill_missing.cu:97-111).
+// Hole filling (fill_missing.cu:35-130) of every frame of the prologue: the
+// holes are listed once, one launch runs all rounds of every frame (a
+// workgroup per frame, kp_fill_all), and one wait reads each frame's outcome.
+// A hole count that stops shrinking is an error as in the reference
+// (fill_missing.cu:97-111).
 static int fill_loop_pro(stsp_batch* bh, const Pro& p, const int* total, hipStream_t s) {
-  const int n = p.nf;
-  std::vector<int> prev(n, -1), live(n, 1);
-  int nlive = n;
-  Pro q = p;  // the launches: every frame, later only those still filling
-  int qtotal[kProKinds];
-  for (int k = 0; k < kProKinds; k++) qtotal[k] = total[k];
-  int chunk = 4;  // most frames finish within a few rounds
-  pro_hole_list(p, total, s);  // every frame's holes at the start of the fill
-  for (int base = 0; base < kFillMax; base += chunk, chunk = kFillChunk) {
-    const int rounds = kFillMax - base < chunk ? kFillMax - base : chunk;
-    pro_fill_rounds(q, qtotal, base, rounds, s);
-    HIP_TRY(hipMemcpyAsync(bh->h_fillcnt, p.fillcnt + base * n, sizeof(int) * rounds * n,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    int any = 0;
-    for (int i = 0; i < n; i++) {
-      if (!live[i]) continue;
-      for (int r = 0; r < rounds; r++) {
-        const int neg = bh->h_fillcnt[r * n + i];
-        if (neg == 0) {
-          live[i] = 0;
-          break;
-        }
-        if (base + r > 0 && neg == prev[i]) return STSP_ERR_FILL;
-        prev[i] = neg;
-      }
-      any |= live[i];
-    }
-    if (!any) return STSP_OK;
-    int m = 0;
-    for (int i = 0; i < n; i++) m += live[i];
-    if (m < nlive) {  // launch over the frames still filling from now on
-      nlive = m;
-      int* off = bh->h_live;
-      int* fmap = bh->h_live + kProKinds * (kProMax + 1);
-      const int* all = bh->h_pro_off;
-      for (int k = 0; k < kProKinds; k++) off[k * (kProMax + 1)] = 0;
-      for (int i = 0, j = 0; i < n; i++) {
-        if (!live[i]) continue;
-        fmap[j] = i;
-        for (int k = 0; k < kProKinds; k++)
-          off[k * (kProMax + 1) + j + 1] = off[k * (kProMax + 1) + j] +
-                                           all[k * (kProMax + 1) + i + 1] -
-                                           all[k * (kProMax + 1) + i];
-        j++;
-      }
-      for (int k = 0; k < kProKinds; k++) qtotal[k] = off[k * (kProMax + 1) + m];
-      HIP_TRY(hipMemcpyAsync(bh->d_live, bh->h_live,
-                             sizeof(int) * (kProKinds * (kProMax + 1) + kProMax),
-                             hipMemcpyHostToDevice, s));
-      q.off = bh->d_live;
-      q.fmap = bh->d_live + kProKinds * (kProMax + 1);
-      q.nf = m;
-    }
-  }
+  pro_hole_list(p, total, s);
+  pro_fill_all(p, s);
+  HIP_TRY(hipMemcpyAsync(bh->h_fillres, p.fillres, sizeof(int) * 2 * p.nall,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < p.nall; i++)
+    if (bh->h_fillres[2 * i + 1]) return STSP_ERR_FILL;
   return STSP_OK;
 }
 
@@ -1096,9 +1033,8 @@ static int prop_prologue_n(stsp_batch* bh, stsp_ctx** cs, int n, const float* co
   p.fr = bh->d_pro;
   p.off = bh->d_pro_off;
   p.nf = n;
-  p.fmap = nullptr;
   p.nall = n;
-  p.fillcnt = bh->d_fillcnt;
+  p.fillres = bh->d_fillres;
   p.negs = bh->d_negs;
   p.hlen = bh->d_hlen;
   pro_shift(p, total, s);

@@ -55,12 +55,7 @@ int pro_blocks(int kind, int W, int H, int np, int nbuf) {
     case kProPix: return div_up((long long)W * H, kBlock);
     case kProLab: return div_up(np, kBlock);
     case kProBuf: return div_up(nbuf, kBlock);
-    case kProTile: return div_up(W, 32) * div_up(H, 32);
-    case kProHole: {
-      const int nb = div_up((long long)W * H, kBlock);
-      return nb < kHoleSub ? nb : kHoleSub;
-    }
-    default: return div_up((long long)((W + 1) / 2) * ((H + 1) / 2), kBlock);  // kProHalf
+    default: return div_up(W, 32) * div_up(H, 32);  // kProTile
   }
 }
 
@@ -187,27 +182,14 @@ void pro_shift(const Pro& p, const int* total, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // hole filling (fill_missing.cu:35-130, 288-441, 617-647).  Round r counts
-// the holes and marks the border (a valid 4-neighbour) at its start, then
-// fills border holes in four checkerboard phases.  A fill never makes a new
-// hole, so the pixels that are holes when a fill starts (listed once, in the
-// frame's `keys` buffer, free after the shift) hold every hole of its
-// rounds: the rounds visit the list, not the frame.  A frame whose count was
-// zero in the round before is finished: its blocks return at once.
+// the holes at its start (stop at zero; a count that did not shrink is the
+// reference's stall, fill_missing.cu:97-111), marks the border holes (a valid
+// 4-neighbour), then fills border holes in four checkerboard phases.  A fill
+// never makes a new hole, so the pixels that are holes when a fill starts
+// (listed once, in the frame's `keys` buffer, free after the shift) hold
+// every hole of its rounds.  One workgroup runs all rounds of a frame, a
+// barrier between the stages.
 // ---------------------------------------------------------------------------
-// adds the block's sum of v to *dst with one atomic (all threads call it)
-__device__ __forceinline__ void block_sum_add(int v, int* dst) {
-  __shared__ int ws[kBlock / 64];
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int n = 0;
-    for (int w = 0; w < kBlock / 64; w++) n += ws[w];
-    if (n) atomicAdd(dst, n);
-  }
-}
-
 __global__ void kp_hole_clear(Pro p) {
   for (int i = threadIdx.x; i < p.nall * kHoleSub; i += blockDim.x) p.hlen[i] = 0;
 }
@@ -241,91 +223,202 @@ void pro_hole_list(const Pro& p, const int* total, hipStream_t s) {
   kp_hole_list<<<total[kProPix], kBlock, 0, s>>>(p);
 }
 
-// The filling rounds are launched over the frames that still have holes
-// (p.fmap lists them; the counts keep the table's frame index).
-__global__ void kp_fill_zero(Pro p, int base, int rounds) {
-  const int n = rounds * p.nall;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p.fillcnt[base * p.nall + i] = 0;
+// One workgroup per frame runs the rounds as a frontier.  Every hole that
+// has a valid 4-neighbour when a round starts is filled in that round (the
+// neighbour stays valid, so its phase finds a candidate), and no other hole
+// is; so round r fills exactly its frontier F_r, and F_{r + 1} is the set of
+// holes still empty next to F_r.  The hole count at round r is the count at
+// the start less the frontiers filled, and a count that does not shrink is an
+// empty frontier.  The frontiers alternate between the two halves of the
+// frame's `keys` buffer (2 npix ints) after the hole sub-lists are read; a
+// bit per pixel in the `border` buffer marks holes already queued.
+constexpr int kFillWaves = 16;
+constexpr int kFillThreads = 64 * kFillWaves;
+
+// appends v (when take) to list at the LDS counter *n, one LDS atomic per wave
+__device__ __forceinline__ void wave_append(int* list, int* n, bool take, int v) {
+  const unsigned long long b = __ballot(take);
+  if (!b) return;
+  const int lane = threadIdx.x & 63;
+  const int first = __ffsll((long long)b) - 1;
+  int base = 0;
+  if (lane == first) base = atomicAdd(n, __popcll(b));
+  base = __shfl(base, first);
+  if (take) list[base + __popcll(b & ((1ull << lane) - 1ull))] = v;
 }
 
-// the blocks of frame fl of a hole-list launch: block hb of nhb visits the
-// sub-lists hb, hb + nhb, ... (entries i of sub-list sub: HOLE_AT(sub, i))
-#define PRO_HOLE                                                           \
-  const int fl = pro_frame(p, kProHole, blockIdx.x);                       \
-  const int f = p.fmap ? p.fmap[fl] : fl;                                  \
-  const ProFrame& F = p.fr[f];                                             \
-  const int* off_h = p.off + kProHole * (kProMax + 1);                     \
-  const int hb = blockIdx.x - off_h[fl], nhb = off_h[fl + 1] - off_h[fl];  \
-  const int* list = reinterpret_cast<const int*>(F.keys);                  \
-  const int subcap = hole_sub_cap(F.npix);                                 \
-  const int* hl = p.hlen + f * kHoleSub;
-
-__global__ void kp_fill_border(Pro p, int r) {
-  PRO_HOLE
-  if (r > 0 && p.fillcnt[(r - 1) * p.nall + f] == 0) return;  // block-uniform
-  const int W = F.W, H = F.H;
-  const int* seg = F.seg;
-  int neg = 0;
-  for (int sub = hb; sub < kHoleSub; sub += nhb) {
-    for (int i = threadIdx.x; i < hl[sub]; i += kBlock) {
-      const int t = list[(size_t)sub * subcap + i];
-      if (seg[t] >= 0) continue;  // filled in an earlier round
-      const int x = t % W, y = t / W;
-      const int N = y > 0 ? seg[t - W] : -1, Wl = x > 0 ? seg[t - 1] : -1;
-      const int S = y < H - 1 ? seg[t + W] : -1, E = x < W - 1 ? seg[t + 1] : -1;
-      F.border[t] = (N >= 0) || (Wl >= 0) || (S >= 0) || (E >= 0);
-      neg++;
+__global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
+  const int f = blockIdx.x;
+  const ProFrame& F = p.fr[f];
+  const int W = F.W, H = F.H, npix = F.npix;
+  int* seg = F.seg;
+  uint32_t* queued = reinterpret_cast<uint32_t*>(F.border);  // npix bits
+  int* const lists = reinterpret_cast<int*>(F.keys);          // 2 npix ints
+  const int subcap = hole_sub_cap(npix);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ int ncur, nnext, nholes;
+  if (tid == 0) {
+    ncur = 0;
+    nholes = 0;
+  }
+  for (int i = tid; i < (npix + 31) / 32; i += kFillThreads) queued[i] = 0u;
+  __syncthreads();
+  // the holes and the first frontier (holes next to a valid pixel), read from
+  // the hole sub-lists into registers before the frontier overwrites them
+  {
+    constexpr int kSubPerWave = kHoleSub / kFillWaves;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kSubPerWave; q++) {
+      const int sub = wv + q * kFillWaves;
+      const int len = p.hlen[f * kHoleSub + sub];
+      cnt += len;
+    }
+    cnt = lane == 0 ? cnt : 0;
+    if (cnt) atomicAdd(&nholes, cnt);
+  }
+  __syncthreads();
+  const int holes0 = nholes;
+  // mark the first frontier's holes in the bitmap (the sub-lists still
+  // occupy the frontier lists' space), then collect the marked bits
+  {
+    constexpr int kSubPerWave = kHoleSub / kFillWaves;
+    for (int q = 0; q < kSubPerWave; q++) {
+      const int sub = wv + q * kFillWaves;
+      const int len = p.hlen[f * kHoleSub + sub];
+      const int* sl = lists + (size_t)sub * subcap;
+      for (int i0 = 0; i0 < len; i0 += 64) {
+        const int i = i0 + lane;
+        int t = -1;
+        bool take = false;
+        if (i < len) {
+          t = sl[i];
+          const int x = t % W, y = t / W;
+          take = (y > 0 && seg[t - W] >= 0) || (x > 0 && seg[t - 1] >= 0) ||
+                 (y < H - 1 && seg[t + W] >= 0) || (x < W - 1 && seg[t + 1] >= 0);
+        }
+        if (take) atomicOr(&queued[t >> 5], 1u << (t & 31));
+      }
     }
   }
-  block_sum_add(neg, p.fillcnt + r * p.nall + f);
-}
-
-__global__ void kp_fill_phase(Pro p, int r, int xm, int ym) {
-  PRO_HOLE
-  if (p.fillcnt[r * p.nall + f] == 0) return;
-  const int W = F.W, H = F.H;
-  int* seg = F.seg;
-  for (int sub = hb; sub < kHoleSub; sub += nhb) {
-    for (int i = threadIdx.x; i < hl[sub]; i += kBlock) {
-      const int idx = list[(size_t)sub * subcap + i];
-      const int x = idx % W, y = idx / W;
-      if ((x & 1) != xm || (y & 1) != ym) continue;
-      if (seg[idx] != -1 || !F.border[idx]) continue;
-      const int N = y > 0 ? seg[idx - W] : -1, Wl = x > 0 ? seg[idx - 1] : -1;
-      const int S = y < H - 1 ? seg[idx + W] : -1, E = x < W - 1 ? seg[idx + 1] : -1;
-      const int cand[4] = {N, S, Wl, E};
-      float best = -999999.0f;
-      int lab = -1;
-      for (int j = 0; j < 4; j++) {
-        int L = cand[j];
-        if (L < 0) continue;
-        bool dup = false;
-        for (int q = 0; q < j; q++) dup |= cand[q] == L;
-        if (dup) continue;
-        // isotropic_space, fill_missing.cu:140-153
-        float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) / (double)(1.0f * (float)W));
-        float dy =
-            (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) / (double)(1.0f * (float)H));
-        float sim = -dx * dx - dy * dy;
-        if (sim > best) {
-          best = sim;
-          lab = L;
+  __syncthreads();
+  // every queued bit is a first-frontier hole: collect them into list 0 from
+  // the bitmap (the sub-lists are no longer needed)
+  {
+    const int nwords = (npix + 31) / 32;
+    for (int w0 = 0; w0 < nwords; w0 += kFillThreads) {
+      const int w = w0 + tid;
+      uint32_t bits = w < nwords ? queued[w] : 0u;
+      const int c = __popc(bits);
+      int x = c;  // inclusive scan of the counts over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      const int total = __shfl(x, 63);
+      int base = 0;
+      if (lane == 0 && total) base = atomicAdd(&ncur, total);
+      base = __shfl(base, 0);
+      int pos = base + x - c;
+      while (bits) {
+        const int bt = __ffs(bits) - 1;
+        bits &= bits - 1;
+        lists[pos++] = w * 32 + bt;
+      }
+    }
+  }
+  __syncthreads();
+  // Each round runs in three stages: the phases (0,0); (0,1) with (1,0),
+  // whose cells are never 4-neighbours of each other, so neither reads the
+  // other's labels; (1,1).  A cell filled in a stage appends its empty,
+  // unqueued 4-neighbours to the next frontier (a hole of this round's
+  // frontier is queued already; a 4-neighbour is never filled in the same
+  // stage).  The frontier counters alternate by round parity: round r reads
+  // fcnt[r & 1] and appends to the other, which round r - 1 read and cleared
+  // after its first stage.
+  __shared__ int fcnt[2];
+  if (tid == 0) {
+    fcnt[0] = ncur;
+    fcnt[1] = 0;
+  }
+  __syncthreads();
+  int holes = holes0, rounds = 0, stalled = 0;
+  int* cur = lists;
+  int* nxt = lists + npix;
+  for (int r = 0; r < kFillMax; r++) {
+    const int n = fcnt[r & 1];  // block-uniform
+    rounds = r + 1;
+    if (holes == 0) break;
+    if (n == 0) {  // the count would not shrink
+      stalled = 1;
+      break;
+    }
+    int* const cnt_next = &fcnt[(r + 1) & 1];
+    for (int stage = 0; stage < 3; stage++) {
+      for (int i0 = 0; i0 < n; i0 += kFillThreads) {
+        const int i = i0 + tid;
+        const int idx = i < n ? cur[i] : 0;
+        const int y = idx / W, x = idx - y * W;
+        const int par = ((x & 1) << 1) | (y & 1);  // (xm, ym)
+        const bool mine = i < n && (stage == 0 ? par == 0 : stage == 1 ? (par == 1 || par == 2)
+                                                                      : par == 3);
+        // N, S, W, E
+        const int nbv[4] = {mine && y > 0 ? seg[idx - W] : -1, mine && y < H - 1 ? seg[idx + W] : -1,
+                            mine && x > 0 ? seg[idx - 1] : -1, mine && x < W - 1 ? seg[idx + 1] : -1};
+        if (mine) {
+          float best = -999999.0f;
+          int lab = -1;
+          for (int j = 0; j < 4; j++) {
+            const int L = nbv[j];
+            if (L < 0) continue;
+            bool dup = false;
+            for (int u = 0; u < j; u++) dup |= nbv[u] == L;
+            if (dup) continue;
+            // isotropic_space, fill_missing.cu:140-153
+            float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) /
+                               (double)(1.0f * (float)W));
+            float dy = (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) /
+                               (double)(1.0f * (float)H));
+            float sim = -dx * dx - dy * dy;
+            if (sim > best) {
+              best = sim;
+              lab = L;
+            }
+          }
+          seg[idx] = lab;
+        }
+        // empty 4-neighbours not yet queued join the next frontier
+        const int nbi[4] = {idx - W, idx + W, idx - 1, idx + 1};
+        const bool ok[4] = {mine && y > 0, mine && y < H - 1, mine && x > 0, mine && x < W - 1};
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+          bool take = false;
+          if (ok[d] && nbv[d] == -1) {
+            const uint32_t bit = 1u << (nbi[d] & 31);
+            take = !(atomicOr(&queued[nbi[d] >> 5], bit) & bit);
+          }
+          wave_append(nxt, cnt_next, take, nbi[d]);
         }
       }
-      seg[idx] = lab;
+      __syncthreads();  // the next stage reads this one's labels
+      // every thread read this round's count before the first barrier, and
+      // the next round appends to it only after two more barriers
+      if (stage == 0 && tid == 0) fcnt[r & 1] = 0;
     }
+    holes -= n;
+    int* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (tid == 0) {
+    p.fillres[2 * f] = rounds;
+    p.fillres[2 * f + 1] = stalled;
   }
 }
-#undef PRO_HOLE
 
-void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s) {
-  kp_fill_zero<<<1, 256, 0, s>>>(p, base, rounds);
-  for (int r = base; r < base + rounds; r++) {
-    kp_fill_border<<<total[kProHole], kBlock, 0, s>>>(p, r);
-    for (int xm = 0; xm < 2; xm++)
-      for (int ym = 0; ym < 2; ym++)
-        kp_fill_phase<<<total[kProHole], kBlock, 0, s>>>(p, r, xm, ym);
-  }
+void pro_fill_all(const Pro& p, hipStream_t s) {
+  kp_fill_all<<<p.nall, kFillThreads, 0, s>>>(p);
 }
 
 // ---------------------------------------------------------------------------

@@ -72,10 +72,10 @@ struct ProFrame {
   uint8_t* border;
   unsigned long long* best;
 };
-enum { kProPix = 0, kProLab, kProBuf, kProTile, kProHalf, kProHole, kProKinds };
+enum { kProPix = 0, kProLab, kProBuf, kProTile, kProKinds };
 // A frame's holes are kept in kHoleSub sub-lists (pixel block b appends to
 // sub-list b % kHoleSub, so the appends of a frame spread over kHoleSub
-// counters); the hole-list kernels have at most kHoleSub blocks per frame.
+// counters).
 constexpr int kHoleSub = 64;
 // partial hole counters per frame: the counting blocks of a frame spread their
 // atomics over kNegSub addresses instead of one
@@ -85,9 +85,8 @@ struct Pro {
   const ProFrame* fr;  // device table
   const int* off;      // [kProKinds][kProMax + 1] over the frames launched
   int nf;              // frames launched (entries of off)
-  const int* fmap;     // launched frame i is table frame fmap[i] (null: i itself)
   int nall;            // frames in the table
-  int* fillcnt;        // [kFillMax][nall]: holes at the start of each round
+  int* fillres;        // [nall][2]: filling rounds run, 1 if the hole count stalled
   int* negs;           // [nall * kNegSub]: holes of the shifted map, kNegSub partial
                        // counters per frame (summed by the host)
   int* hlen;           // [nall * kHoleSub]: lengths of each frame's hole sub-lists
@@ -99,7 +98,7 @@ int pro_blocks(int kind, int W, int H, int np, int nbuf);
 // the whole prologue except the hole-filling rounds
 void pro_shift(const Pro& p, const int* total, hipStream_t s);
 void pro_hole_list(const Pro& p, const int* total, hipStream_t s);
-void pro_fill_rounds(const Pro& p, const int* total, int base, int rounds, hipStream_t s);
+void pro_fill_all(const Pro& p, hipStream_t s);
 void pro_disconnected(const Pro& p, const int* total, hipStream_t s);
 void pro_finish(const Pro& p, const int* total, hipStream_t s);
 // colour preprocessing (k_color.hip)
