@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   constexpr int LCAP = ((TW + 2 * RB0 + 1) / 2) * ((TH + 2 * RB0 + 1) / 2);
   constexpr int LISTB = 4 * LCAP * 2;
   constexpr int POOLB = LISTB > (int)sizeof(MoveTable) ? LISTB : (int)sizeof(MoveTable);
-  __shared__ int lab[LH * LW];
+  __shared__ __align__(16) int lab[LH * LW];
   // the phase lists, and after the last phase the table of statistics moves
   __shared__ __align__(16) unsigned char pool[POOLB];
   unsigned short (*list)[LCAP] = reinterpret_cast<unsigned short (*)[LCAP]>(pool);
@@ -161,22 +161,39 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
       // edges first.
       const int rb = 4 * (S - 1 - s) + 3;
       static_assert(4 * (S - 1) + 3 < R, "the flag region keeps one cell of margin");
-      const int bx0 = R - rb, by0 = R - rb, bw = TW + 2 * rb, bh = TH + 2 * rb;
-      for (int i = tid; i < bw * bh; i += NT) {
-        const int lx = bx0 + i % bw, ly = by0 + i / bw;
-        const int gx = x0 + lx, gy = y0 + ly;
-        const int li = ly * LW + lx;
-        const int C = lab[li];
-        const bool diff = (C != lab[li - LW]) | (C != lab[li + LW]) | (C != lab[li + 1]) |
-                          (C != lab[li - 1]);
-        const bool edge = gy < 1 || gx < 1 || gy >= H - 1 || gx >= W - 1;
-        const bool inside = gx >= 0 && gx < W && gy >= 0 && gy < H;
-        const int p = ((gx & 1) << 1) | (gy & 1);
-        const int rp = 4 * (S - 1 - s) + (3 - p);
-        const bool inreg = lx >= R - rp && lx < R + TW + rp && ly >= R - rp && ly < R + TH + rp;
-        if (inside && (edge || diff) && inreg) {
-          const int pos = atomicAdd(&cnt[p], 1);
-          list[p][pos] = (unsigned short)li;
+      static_assert(LW % 4 == 0 && R % 4 == 0, "chunks of 4 cells are 16-byte aligned");
+      // chunks of 4 cells (x = 4 cx .. 4 cx + 3 of a row) covering the region
+      // rows by0 .. by0 + bh - 1 and columns R - rb .. R + TW + rb - 1
+      const int by0 = R - rb, bh = TH + 2 * rb;
+      const int cx0 = (R - rb) >> 2, ncx = ((R + TW + rb + 3) >> 2) - cx0;
+      for (int i = tid; i < ncx * bh; i += NT) {
+        const int ly = by0 + i / ncx, x = 4 * (cx0 + i % ncx);
+        const int li = ly * LW + x;
+        const int4 c4 = *reinterpret_cast<const int4*>(lab + li);
+        const int4 u4 = *reinterpret_cast<const int4*>(lab + li - LW);
+        const int4 d4 = *reinterpret_cast<const int4*>(lab + li + LW);
+        const int lft = lab[li - 1], rgt = lab[li + 4];
+        const int C[4] = {c4.x, c4.y, c4.z, c4.w};
+        const int U[4] = {u4.x, u4.y, u4.z, u4.w};
+        const int D[4] = {d4.x, d4.y, d4.z, d4.w};
+        const int L[4] = {lft, c4.x, c4.y, c4.z};
+        const int Rt[4] = {c4.y, c4.z, c4.w, rgt};
+        const int gy = y0 + ly;
+        const bool rowin = gy >= 0 && gy < H;
+        const bool rowedge = gy < 1 || gy >= H - 1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int lx = x + j, gx = x0 + lx;
+          const bool diff = (C[j] != U[j]) | (C[j] != D[j]) | (C[j] != L[j]) | (C[j] != Rt[j]);
+          const bool edge = rowedge || gx < 1 || gx >= W - 1;
+          const bool inside = rowin && gx >= 0 && gx < W;
+          const int p = ((j & 1) << 1) | (gy & 1);  // x0 + x is even
+          const int rp = 4 * (S - 1 - s) + (3 - p);
+          const bool inreg = lx >= R - rp && lx < R + TW + rp && ly >= R - rp && ly < R + TH + rp;
+          if (inside && (edge || diff) && inreg) {
+            const int pos = atomicAdd(&cnt[p], 1);
+            list[p][pos] = (unsigned short)(li + j);
+          }
         }
       }
       __syncthreads();
