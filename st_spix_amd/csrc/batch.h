@@ -129,7 +129,7 @@ void b_relabel(const Batch& b, int cur, hipStream_t s);
 void b_count_living(const Batch& b, int slot, hipStream_t s);
 // pieces of the split step that live beside the statistics kernels
 void b_sm_begin(const Batch& b, hipStream_t s);               // clear split/merge state, work lists
-void b_halves_stats(const Batch& b, int cur, hipStream_t s);  // statistics of the split halves
+void b_moved_stats(const Batch& b, int cur, hipStream_t s);   // statistics of the moved split parts
 void b_scan(const Batch& b, int which, hipStream_t s);        // per-frame exclusive scan
 
 }  // namespace stsp

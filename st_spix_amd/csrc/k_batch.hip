@@ -35,8 +35,9 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
   }
 }
 
-// which: 0 = live labels into st (limit nbuf); 1 = split halves into sst
-// (limit nsm, ninvalid from the shifted map); 2 = live labels into sst.
+// Statistics of the live labels (limit nbuf) into st, or with MOVED those
+// of the parts that a split search moved to new labels (the halves map's
+// labels above max_spix) into sst.
 //
 // A workgroup covers a kStatTile x kStatTile tile: wave w its rows 8w..8w+7,
 // lane l the four pixels at column 4*(l&7) of row l>>3.  For each distinct
@@ -45,12 +46,16 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 // atomics.  Position moments are summed relative to the tile origin in int32
 // and widened exactly: sum x = n*x0 + sum dx, sum x^2 = n*x0^2 + 2*x0*sum dx
 // + sum dx^2, sum xy = n*x0*y0 + x0*sum dy + y0*sum dx + sum dx*dy.
-// SQ: also gather the colour squares (only the split and merge passes read
-// them; the live-label statistics never do).
-// SUMS = false: bounding boxes (and the halves copy) only, for the pass
-// before a split, when the sums are already current
-template <bool BBOX, bool SQ, bool SUMS = true>
-__global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int which) {
+// SQ: also gather the colour squares (split and merge read them; the sweeps
+// keep them current with the other sums).
+// BBOX: also the bounding boxes, the halves map (a copy of the labels) and,
+// for propagated frames, the count of pixels whose shifted label is invalid
+// (ninvalid): the pass before a split.
+// SUMS = false: only what BBOX and SQ add, when the other sums are already
+// current (the sweeps keep them)
+template <bool BBOX, bool SQ, bool SUMS = true, bool MOVED = false>
+__global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
+  static_assert(!MOVED || (SUMS && SQ && !BBOX), "the moved parts take every sum");
   const int f = frame_of(b.off.st, b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
   const int W = F.W, H = F.H;
@@ -60,10 +65,11 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
   const int lane = threadIdx.x & 63;
   const int lx = 4 * (lane & 7), ly = 8 * (threadIdx.x >> 6) + (lane >> 3);
   const int y = ty0 + ly;
-  const int* labs = which == 1 ? F.halves : F.seg[cur];
-  LabelStats st = which == 0 ? F.st : F.sst;
-  const int nlab = which == 0 ? F.nbuf : F.nsm;
-  const int* shifted = (which == 1 && F.prop_mode) ? F.shifted : nullptr;
+  const int* labs = MOVED ? F.halves : F.seg[cur];
+  LabelStats st = MOVED ? F.sst : F.st;
+  const int nlab = MOVED ? F.nsm : F.nbuf;
+  const int lo = MOVED ? *F.max_spix + 1 : 0;  // the moved parts: k + max_spix + 1
+  const int* shifted = ((BBOX || MOVED) && F.prop_mode) ? F.shifted : nullptr;
   int k[4];
   float col[4][3];
   int inv[4];
@@ -80,16 +86,16 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       // the pass before a split also starts the halves map (a copy of the
       // labels; the split search then writes the moving parts)
       if (BBOX) F.halves[t] = v;
-      if (v >= 0 && v < nlab) {
+      if (v >= lo && v < nlab) {
         k[j] = v;
         pend |= 1u << j;
-        if (SUMS) {
+        if (SUMS || SQ) {
           const float* p = F.img + 3 * t;
           col[j][0] = p[0];
           col[j][1] = p[1];
           col[j][2] = p[2];
-          if (shifted) inv[j] = shifted[t] < 0;
         }
+        if (shifted) inv[j] = shifted[t] < 0;
       }
     }
   }
@@ -108,18 +114,18 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
         pend &= ~(1u << j);
         const int dx = lx + j;
         cnt += 1;
+        ninv += inv[j];
         if (SUMS) {
           sdx += dx;
           sdy += ly;
           sxx += dx * dx;
           sxy += dx * ly;
           syy += ly * ly;
-          ninv += inv[j];
+        }
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            S[c] += fx_color(col[j][c]);
-            if (SQ) Q[c] += fx_square(col[j][c]);
-          }
+        for (int c = 0; c < 3; c++) {
+          if (SUMS) S[c] += fx_color(col[j][c]);
+          if (SQ) Q[c] += fx_square(col[j][c]);
         }
         if (BBOX) {
           mnx = min(mnx, tx0 + dx);
@@ -128,6 +134,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       }
     }
     int mny = cnt ? y : 0x7fffffff, mxy = cnt ? y : -1;
+    if (shifted) ninv = wave_sum_to63(ninv);
     if (SUMS) {
       cnt = wave_sum_to63(cnt);
       sdx = wave_sum_to63(sdx);
@@ -135,12 +142,11 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       sxx = wave_sum_to63(sxx);
       sxy = wave_sum_to63(sxy);
       syy = wave_sum_to63(syy);
-      if (shifted) ninv = wave_sum_to63(ninv);
+    }
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        S[c] = wave_sum64_to63(S[c]);
-        if (SQ) Q[c] = wave_sum64_to63(Q[c]);
-      }
+    for (int c = 0; c < 3; c++) {
+      if (SUMS) S[c] = wave_sum64_to63(S[c]);
+      if (SQ) Q[c] = wave_sum64_to63(Q[c]);
     }
     if (BBOX) {
       mnx = wave_min_to63(mnx);
@@ -149,6 +155,11 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur, int whic
       mxy = wave_max_to63(mxy);
     }
     if (lane == 63 && !SUMS && BBOX) {
+      if (SQ) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) atomic_add_i64(st.Q + 3 * u + c, Q[c]);
+      }
+      if (ninv) atomicAdd(st.ninvalid + u, ninv);
       atomicMin(st.bbox + 4 * u, mnx);
       atomicMax(st.bbox + 4 * u + 1, mxx);
       atomicMin(st.bbox + 4 * u + 2, mny);
@@ -308,15 +319,16 @@ void b_param_only(const Batch& b, hipStream_t s) {
 void b_update_params(const Batch& b, int cur, bool bbox, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, bbox ? 1 : 0);
   if (bbox)
-    kb_stats_accum<true, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<true, true><<<b.st_blocks, 256, 0, s>>>(b, cur);
   else
-    kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+    kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
 __global__ void kb_bbox_clear(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
+  F.st.ninvalid[k] = 0;
   int* bb = F.st.bbox + 4 * k;
   bb[0] = 0x7fffffff;
   bb[1] = -1;
@@ -325,17 +337,18 @@ __global__ void kb_bbox_clear(Batch b) {
 }
 
 // Before a split: the label sums are current (the sweeps move them, split and
-// merge combine them), so the pass gathers only the bounding boxes and
-// starts the halves map; then the parameters are updated.
+// merge combine them), so the pass gathers only the bounding boxes and the
+// invalid counts (which the sweeps do not keep) and starts the halves map;
+// then the parameters are updated.
 void b_prepare_split(const Batch& b, int cur, hipStream_t s) {
   kb_bbox_clear<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  kb_stats_accum<true, false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+  kb_stats_accum<true, false, false><<<b.st_blocks, 256, 0, s>>>(b, cur);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
 void b_param_summary(const Batch& b, int cur, hipStream_t s) {
   kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
-  kb_stats_accum<false, false><<<b.st_blocks, 256, 0, s>>>(b, cur, 0);
+  kb_stats_accum<false, false><<<b.st_blocks, 256, 0, s>>>(b, cur);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 1);
 }
 
@@ -832,8 +845,8 @@ void b_sm_begin(const Batch& b, hipStream_t s) {
   kb_reset_work<<<1, 64, 0, s>>>(b);
 }
 
-void b_halves_stats(const Batch& b, int cur, hipStream_t s) {
-  kb_stats_accum<false, true><<<b.st_blocks, 256, 0, s>>>(b, cur, 1);
+void b_moved_stats(const Batch& b, int cur, hipStream_t s) {
+  kb_stats_accum<false, true, true, true><<<b.st_blocks, 256, 0, s>>>(b, cur);
 }
 
 void b_scan(const Batch& b, int which, hipStream_t s) {

@@ -964,6 +964,20 @@ __global__ void kb_split_eval(Batch b) {
   const LabelStats& st = F.st;
   const LabelStats& sst = F.sst;
   const NigConsts& nc = F.nc;
+  // the part that stays: the label's statistics (current before the split:
+  // the sweeps keep the sums, the pass that found the boxes gathered the
+  // invalid counts) less the part that moves (kb_stats_accum over the
+  // halves map; zero where nothing moves)
+  if (k > ms) return;  // no pixels; sst[k] of such k is a moving part, read below
+  sst.count[k] = st.count[k] - sst.count[s];
+  sst.ninvalid[k] = st.ninvalid[k] - sst.ninvalid[s];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    sst.S[3 * k + c] = st.S[3 * k + c] - sst.S[3 * s + c];
+    sst.Q[3 * k + c] = st.Q[3 * k + c] - sst.Q[3 * s + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) sst.X[5 * k + c] = st.X[5 * k + c] - sst.X[5 * s + c];
   int count_f = P.count[k], count_k = sst.count[k], count_s = sst.count[s];
   if (count_f < 1 || count_k < 1 || count_s < 1) return;
   if (count_f != count_k + count_s) return;
@@ -1064,13 +1078,13 @@ __global__ void kb_split_apply(Batch b, int cur) {
   }
 }
 
-// Label statistics after a split without a recount: the halves pass
-// gathered both parts of every label (k and k + max_spix + 1) with exact
-// integer sums, and the labels that did not split kept their pixels, so an
-// accepted label takes the sums of the part that stays and its new label
-// those of the part that moves (the assignment of kb_split_apply).  Must run
-// before kb_add_max changes max_spix.  (The invalid counts are not kept for
-// live labels.)
+// Label statistics after a split without a recount: the pass over the
+// halves map and kb_split_eval left both parts of every label (k and
+// k + max_spix + 1) as exact integer sums, and the labels that did not split
+// kept their pixels, so an accepted label takes the sums of the part that
+// stays and its new label those of the part that moves (the assignment of
+// kb_split_apply).  Must run before kb_add_max changes max_spix.  (The
+// invalid counts are not kept for live labels.)
 __global__ void kb_split_stats(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
@@ -1110,8 +1124,9 @@ __global__ void kb_add_max(Batch b) {
 // The three BFS kernels write disjoint pixels (different labels): the few
 // long workgroup searches run on two side streams beside the wave searches
 // (all on the main stream when the batch has no side streams).
-// Expects the halves map to equal the labels: the statistics pass with
-// bounding boxes that precedes every split writes it.
+// Expects the halves map to equal the labels and the label statistics to
+// hold the invalid counts: the statistics pass with bounding boxes that
+// precedes every split writes them.
 void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideStream& side) {
   b_sm_begin(b, s);
   kb_split_list<<<b.lab_blocks, kBlock, 0, s>>>(b);
@@ -1129,7 +1144,7 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
     (void)hipStreamWaitEvent(s, side.join, 0);
     (void)hipStreamWaitEvent(s, side.join2, 0);
   }
-  b_halves_stats(b, cur, s);
+  b_moved_stats(b, cur, s);
   kb_split_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   b_scan(b, 0, s);
   kb_split_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
