@@ -86,7 +86,10 @@ struct BatchOff {
 };
 
 constexpr int kOffArrays = 4 + kSweepShapes;
-constexpr int kWorkLists = 3, kWorkCounters = 2 * kWorkLists;  // pix, lab, sm, tile[], st
+// [2i] length and [2i + 1] cursor of list i; [kWorkMidLight] the length of
+// the block list's part filled from its end
+constexpr int kWorkLists = 3, kWorkMidLight = 2 * kWorkLists, kWorkCounters = kWorkMidLight + 1;
+constexpr int kMidHeavy = 200 * 200;  // block-list boxes searched first (pixels of box area)
 constexpr int kStatTile = 32;
 
 struct Batch {

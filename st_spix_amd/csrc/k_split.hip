@@ -10,7 +10,10 @@ namespace stsp {
 // (at most 128 x 128, the wave kernel's register rows) go to the wave list
 // work[0..); boxes up to 448 x 384 or 384 x 448 that also fit the frame's
 // search window to the block list work[work_cap..) (searched whole by the
-// 8-wave kernel); the others to the big-block list work[2 work_cap..).
+// 8-wave kernel); the others to the big-block list work[2 work_cap..).  The
+// block list holds its larger boxes (area at least kMidHeavy pixels) from
+// its start and the others from its end, so the 8-wave kernel takes the
+// longest searches first and the short ones fill in behind them.
 
 __global__ void kb_split_list(Batch b) {
   LAB_PROLOGUE
@@ -26,8 +29,13 @@ __global__ void kb_split_list(Batch b) {
     if (pos < b.work_cap) b.work[pos] = (f << 24) | k;
   } else if (((bh <= kMidRows && bw <= kMidCols) || (bw <= kMidRows && bh <= kMidCols)) &&
              bh <= F.win_h && bw <= F.win_w) {
-    const int pos = atomicAdd(b.work_n + 2, 1);
-    if (pos < b.work_cap) b.work[b.work_cap + pos] = (f << 24) | k;
+    if (bw * bh >= kMidHeavy) {
+      const int pos = atomicAdd(b.work_n + 2, 1);
+      if (pos < b.work_cap) b.work[b.work_cap + pos] = (f << 24) | k;
+    } else {
+      const int pos = atomicAdd(b.work_n + kWorkMidLight, 1);
+      if (pos < b.work_cap) b.work[2 * b.work_cap - 1 - pos] = (f << 24) | k;
+    }
   } else {
     const int pos = atomicAdd(b.work_n + 4, 1);
     if (pos < b.work_cap) b.work[2 * b.work_cap + pos] = (f << 24) | k;
@@ -60,6 +68,19 @@ __device__ __forceinline__ int seed_of(const FrameDesc& F, const int* seg, int k
 // reached from seed 2 at a level no earlier than seed 1.  NK collects the
 // pixels that seed 2 reaches while seed 1 already has them (d2 >= d1), so
 // the new label takes V1 & ~NK.
+//
+// The band searches (bfs_bands) grow the two regions R1 = {d1 <= d2} and
+// R2 = {d2 < d1} instead, each only into pixels neither holds yet (a pixel
+// both reach at one level joins R1):
+//   N1 = grow(R1) & M & ~(R1 | R2),  N2 = grow(R2) & M & ~(R1 | R2) & ~N1.
+// Both sets hold the predecessor of each of their pixels on a shortest path
+// from their seed (if d1(p) <= d2(p) then d1(q) = d1(p) - 1 <= d2(q) for
+// that q; likewise with < for R2), so level L adds exactly the pixels with
+// min(d1, d2) = L.  When the seeds share a 4-connected piece, the new label
+// takes R2 (d2 < d1); otherwise seed 2 reaches no pixel of seed 1's piece
+// and the new label takes all of R1.  The seeds share a piece exactly when
+// R1 and R2 hold 4-adjacent pixels.  Each region grows only in the cells its
+// front passes, about half of those either front passes.
 // ---------------------------------------------------------------------------
 template <int RC, int WD>
 __device__ __forceinline__ void bits_expand(const uint64_t (&V)[RC][WD],
@@ -224,6 +245,7 @@ struct BandEdge {
   uint64_t v[2][NBANDS][2][kHalo][2][WDMAX];  // [buf][band][top|bottom][row][seed][word]
   uint32_t g[2][NW];                          // [buf][wave]: own rows grew
   uint32_t esc;                               // seed 1's component leaves the window
+  uint32_t same;                              // the seeds lie in one piece
 };
 typedef BandEdge<16, 8, 8> BigEdge;
 
@@ -244,7 +266,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
                                           uint64_t* __restrict__ mlds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nrow = TR ? bw : bh;  // rows of the band structure
-  uint64_t M[MLDS ? 1 : RB][MLDS ? 1 : WD], V1[RB][WD], V2[RB][WD], NK[RB][WD];
+  uint64_t M[MLDS ? 1 : RB][MLDS ? 1 : WD], V1[RB][WD], V2[RB][WD];
   // the mask word of chunk c, word w for this lane
 #define STSP_MASK(c, w) \
   (MLDS ? mlds[((NW * (c) + wv) * WD + (w)) * 64 + lane] : M[MLDS ? 0 : (c)][MLDS ? 0 : (w)])
@@ -256,7 +278,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
 #pragma unroll
     for (int w = 0; w < WD; w++) {
       uint64_t m = 0;
-      V1[c][w] = V2[c][w] = NK[c][w] = 0;
+      V1[c][w] = V2[c][w] = 0;
       const int col = 64 * w + lane;
       if (TR && row0 < nrow) {
         // lane = box column x, bit r = box row 64 w + r: each load reads 64
@@ -327,50 +349,59 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     }
     return out;
   };
-  uint32_t act = 0;
+  // per-seed activity: bit c * WD + w of act1 (act2) marks the cells where
+  // region 1 (2) may grow at the next level
+  uint32_t act1 = 0, act2 = 0;
 #pragma unroll
   for (int c = 0; c < RB; c++)
 #pragma unroll
-    for (int w = 0; w < WD; w++)
-      if (__ballot((V1[c][w] | V2[c][w]) != 0) != 0) act |= 1u << (c * WD + w);
-  act = spread(act);
+    for (int w = 0; w < WD; w++) {
+      if (__ballot(V1[c][w] != 0) != 0) act1 |= 1u << (c * WD + w);
+      if (__ballot(V2[c][w] != 0) != 0) act2 |= 1u << (c * WD + w);
+    }
+  act1 = spread(act1);
+  act2 = spread(act2);
   for (int round = 0;; round++) {
     uint32_t grew = 0;
 #pragma unroll 1
     for (int j = 0; j < kHalo; j++) {
-      uint32_t ch = 0;
+      uint32_t ch1 = 0, ch2 = 0;
 #pragma unroll
       for (int c = 0; c < RB; c++) {
         uint64_t N1[WD], N2[WD];
 #pragma unroll
         for (int w = 0; w < WD; w++) {
           N1[w] = N2[w] = 0;
-          if (!((act >> (c * WD + w)) & 1u)) continue;  // wave-uniform
-          uint64_t h1 = (V1[c][w] << 1) | (V1[c][w] >> 1), h2 = (V2[c][w] << 1) | (V2[c][w] >> 1);
-          if (w > 0) {
-            h1 |= V1[c][w - 1] >> 63;
-            h2 |= V2[c][w - 1] >> 63;
+          const uint32_t bit = 1u << (c * WD + w);
+          if (!((act1 | act2) & bit)) continue;  // wave-uniform
+          const uint64_t m = STSP_MASK(c, w) & ~(V1[c][w] | V2[c][w]);
+          if (act1 & bit) {  // wave-uniform
+            uint64_t h = (V1[c][w] << 1) | (V1[c][w] >> 1);
+            if (w > 0) h |= V1[c][w - 1] >> 63;
+            if (w + 1 < WD) h |= V1[c][w + 1] << 63;
+            N1[w] = (h | lane_up(V1[c][w], 0ull) | lane_down(V1[c][w], 0ull)) & m;
           }
-          if (w + 1 < WD) {
-            h1 |= V1[c][w + 1] << 63;
-            h2 |= V2[c][w + 1] << 63;
+          if (act2 & bit) {
+            uint64_t h = (V2[c][w] << 1) | (V2[c][w] >> 1);
+            if (w > 0) h |= V2[c][w - 1] >> 63;
+            if (w + 1 < WD) h |= V2[c][w + 1] << 63;
+            N2[w] = (h | lane_up(V2[c][w], 0ull) | lane_down(V2[c][w], 0ull)) & m;
           }
-          const uint64_t m = STSP_MASK(c, w);
-          N1[w] = (h1 | lane_up(V1[c][w], 0ull) | lane_down(V1[c][w], 0ull)) & m & ~V1[c][w];
-          N2[w] = (h2 | lane_up(V2[c][w], 0ull) | lane_down(V2[c][w], 0ull)) & m & ~V2[c][w];
         }
 #pragma unroll
         for (int w = 0; w < WD; w++) {
-          if (!((act >> (c * WD + w)) & 1u)) continue;  // N1 = N2 = 0 there
+          const uint32_t bit = 1u << (c * WD + w);
+          if (!((act1 | act2) & bit)) continue;  // N1 = N2 = 0 there
+          const uint64_t n2 = N2[w] & ~N1[w];  // a tie stays with seed 1
           V1[c][w] |= N1[w];
-          NK[c][w] |= N2[w] & V1[c][w];
-          V2[c][w] |= N2[w];
-          const uint64_t any = N1[w] | N2[w];
-          if (__ballot(any != 0) != 0) ch |= 1u << (c * WD + w);
-          if (__ballot(own && any != 0) != 0) grew = 1;
+          V2[c][w] |= n2;
+          if (__ballot(N1[w] != 0) != 0) ch1 |= bit;
+          if (__ballot(n2 != 0) != 0) ch2 |= bit;
+          if (__ballot(own && (N1[w] | n2) != 0) != 0) grew = 1;
         }
       }
-      act = spread(ch);
+      act1 = spread(ch1);
+      act2 = spread(ch2);
     }
     // publish the outer own rows: lanes 4..7 (top) and 56..59 (bottom)
     const int wb = round & 1;
@@ -394,7 +425,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     // lanes 0..3 take the band above's bottom rows, lanes 60..63 the band
     // below's top rows (none above band 0 or below the last band: zeros)
     const bool up = lane < kHalo, dn = lane >= 64 - kHalo;
-    uint32_t ch = 0;
+    uint32_t ch1 = 0, ch2 = 0;
 #pragma unroll
     for (int c = 0; c < RB; c++) {
       const int b = NW * c + wv;
@@ -407,29 +438,55 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
         const uint64_t e2 = ok ? E.v[wb][ok ? nb : 0][up ? 1 : 0][ok ? r : 0][1][w] : 0ull;
         const uint64_t n1 = (up || dn) ? e1 : V1[c][w];
         const uint64_t n2 = (up || dn) ? e2 : V2[c][w];
-        if (__ballot(n1 != V1[c][w] || n2 != V2[c][w]) != 0) ch |= 1u << (c * WD + w);
+        if (__ballot(n1 != V1[c][w]) != 0) ch1 |= 1u << (c * WD + w);
+        if (__ballot(n2 != V2[c][w]) != 0) ch2 |= 1u << (c * WD + w);
         V1[c][w] = n1;
         V2[c][w] = n2;
       }
     }
-    // cells that changed at the last level or took new halo rows
-    act |= spread(ch);
+    // cells that changed at the last level or took new halo rows (a halo row
+    // that changed may let either region grow into the cell's own rows)
+    act1 |= spread(ch1 | ch2);
+    act2 |= spread(ch1 | ch2);
   }
+  // Seed 1's and seed 2's regions are adjacent exactly when the seeds lie in
+  // one 4-connected piece of the label (a piece split in two parts has a
+  // pair of adjacent pixels across the parts).  At the fixed point the halo
+  // rows equal the neighbouring bands' rows, so own rows see every neighbour.
+  {
+    bool adj = false;
+#pragma unroll
+    for (int c = 0; c < RB; c++)
+#pragma unroll
+      for (int w = 0; w < WD; w++) {
+        uint64_t h = (V1[c][w] << 1) | (V1[c][w] >> 1);
+        if (w > 0) h |= V1[c][w - 1] >> 63;
+        if (w + 1 < WD) h |= V1[c][w + 1] << 63;
+        h |= lane_up(V1[c][w], 0ull) | lane_down(V1[c][w], 0ull);
+        adj = adj || (own && (h & V2[c][w]) != 0);
+      }
+    if (tid == 0) E.same = 0;
+    __syncthreads();
+    if (__ballot(adj) != 0 && lane == 0) E.same = 1;
+    __syncthreads();
+  }
+  const bool same = __builtin_amdgcn_readfirstlane(E.same) != 0;
   if (!TR && esc_sides) {
-    // does seed 1's component touch an open window edge?  Loads outside the
-    // window stay inside the label's box (esc_sides marks only such edges)
+    // does seed 1's component (region 1, with region 2 when the seeds share
+    // a piece) touch an open window edge?  Loads outside the window stay
+    // inside the label's box (esc_sides marks only such edges)
     bool esc = false;
 #pragma unroll
     for (int c = 0; c < RB; c++) {
       const int y = kOwn * (NW * c + wv) - kHalo + lane;
       const bool mine = own && y < bh;
-      if ((esc_sides & kEscLeft) && mine && (V1[c][0] & 1ull))
+      if ((esc_sides & kEscLeft) && mine && (((same ? V1[c][0] | V2[c][0] : V1[c][0])) & 1ull))
         esc = esc || seg[(by0 + y) * W + bx0 - 1] == k;
       if ((esc_sides & kEscRight) && mine) {
         const int wl = (bw - 1) >> 6;
-        uint64_t word = V1[c][0];
+        uint64_t word = same ? V1[c][0] | V2[c][0] : V1[c][0];
 #pragma unroll
-        for (int w = 1; w < WD; w++) word = w == wl ? V1[c][w] : word;
+        for (int w = 1; w < WD; w++) word = w == wl ? (same ? V1[c][w] | V2[c][w] : V1[c][w]) : word;
         if ((word >> ((bw - 1) & 63)) & 1ull) esc = esc || seg[(by0 + y) * W + bx0 + bw] == k;
       }
       // top edge: window row 0 is own row kHalo of band 0; bottom edge: row
@@ -447,7 +504,7 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
           const int col = 64 * w + lane;
           const int v = col < bw ? seg[gy * W + bx0 + col] : -1;
           const uint64_t outside = __ballot(v == k);
-          const uint64_t inside = lane_get(V1[c][w], ln);
+          const uint64_t inside = lane_get(same ? V1[c][w] | V2[c][w] : V1[c][w], ln);
           esc = esc || (outside & inside) != 0;
         }
       }
@@ -461,7 +518,9 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
     const int row0 = kOwn * (NW * c + wv) - kHalo;
 #pragma unroll
     for (int w = 0; w < WD; w++) {
-      const uint64_t K = V1[c][w] & ~NK[c][w];
+      // the moving part: region 2 when the seeds share a piece, else all of
+      // seed 1's piece (seed 2 never reaches it)
+      const uint64_t K = same ? V2[c][w] : V1[c][w];
       if (__ballot(own && K != 0) == 0) continue;  // wave-uniform: nothing moves here
       if (TR) {
         // this lane's own column, rows 64 w .. 64 w + 63: the lanes of one
@@ -793,8 +852,9 @@ __device__ __forceinline__ void bfs_global(const int* __restrict__ seg, int W, i
 }
 
 // One workgroup of NW waves per label of work list LIST (the big-box list:
-// NW = 8, a window around seed 1 when the box exceeds it, then the whole-box
-// fallbacks).
+// NW = 8; a box that fits the bands only transposed is searched so, whole;
+// else a window around seed 1 when the box exceeds the bands, then the
+// whole-box fallbacks).
 template <int NW, int LIST, int MAXR, int MAXC, class Edge>
 __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int direction, Edge& edge,
                                                 uint64_t* sM, uint64_t* sV1, uint64_t* sV2,
@@ -827,6 +887,24 @@ __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int dir
     // the band window: the whole box if it fits, else a window around seed 1
     const int win_h = min(MAXR, __builtin_amdgcn_readfirstlane(F.win_h));
     const int win_w = min(MAXC, __builtin_amdgcn_readfirstlane(F.win_w));
+    if (!(bh <= win_h && bw <= win_w) && bw <= win_h && bh <= win_w) {
+      // too wide for the bands, but it fits them transposed: searched whole
+      // (box columns as band rows), no window and no fallback
+      static_assert(NW == 8, "the big-box kernel has 8 waves");
+#define STSP_BANDS_T(RB, WD)                                                                  \
+  (void)bfs_bands<NW, RB, WD, true, Edge, true>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, \
+                                                edge, 0, sM)
+      if (bw <= NW * kOwn) {
+        if (bh <= 256) STSP_BANDS_T(1, 4);
+        else STSP_BANDS_T(1, 8);
+      } else {
+        if (bh <= 256) STSP_BANDS_T(2, 4);
+        else STSP_BANDS_T(2, 8);
+      }
+#undef STSP_BANDS_T
+      if (tid == 0) atomicAdd(F.search_count + kSearchBand, 1ull);
+      continue;
+    }
     int wx0 = bx0, wy0 = by0, ww = bw, wh = bh, sides = 0;
     if (bh > win_h) {
       wh = win_h;
@@ -885,14 +963,19 @@ __global__ void __launch_bounds__(512) kb_bfs_mid(Batch b, int cur, int directio
   __shared__ MidEdge edge;
   __shared__ int item;
   const int tid = threadIdx.x;
-  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
+  // larger boxes from the list's start, then the others from its end (both
+  // parts together never exceed the list: one entry per label)
+  const int n_heavy = __builtin_amdgcn_readfirstlane(min(b.work_n[2], b.work_cap));
+  const int n_items = __builtin_amdgcn_readfirstlane(
+      min(n_heavy + min(b.work_n[kWorkMidLight], b.work_cap), b.work_cap));
   while (true) {
     __syncthreads();
     if (tid == 0) item = atomicAdd(b.work_n + 3, 1);
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(item);
     if (it >= n_items) break;
-    const int wv = __builtin_amdgcn_readfirstlane(b.work[b.work_cap + it]);
+    const int slot = it < n_heavy ? b.work_cap + it : 2 * b.work_cap - 1 - (it - n_heavy);
+    const int wv = __builtin_amdgcn_readfirstlane(b.work[slot]);
     const int f = wv >> 24, k = wv & 0xffffff;
     const FrameDesc& F = b.fr[f];
     const int* seg = F.seg[cur];

@@ -81,9 +81,13 @@ def test_split_search_paths_churn(orc):
     assert counts["window"] + counts["whole"] > 0
 
 
-def test_split_search_large_boxes(orc):
+@pytest.mark.parametrize("search", [(16, 448, 384), (16, 896, 256)])
+def test_split_search_large_boxes(orc, search):
     """Few, large superpixels: boxes of a few hundred pixels, searched whole by
-    the 8-wave kernel in both orientations (1 to 6 words per band row)."""
-    counts = run_video(orc, 4, 256, 480, 6, 90, 20, seed=8, search=(16, 448, 384))
-    print("large", counts)
+    the 8-wave kernel in both orientations (1 to 6 words per band row).  With
+    the narrow window (256 columns) the boxes wider than it are searched
+    whole and transposed by the big-box kernel, in one or two bands per wave
+    (box widths up to 448, then up to 480)."""
+    counts = run_video(orc, 4, 256, 480, 6, 90, 20, seed=8, search=search)
+    print("large", search, counts)
     assert counts["band"] > 0
