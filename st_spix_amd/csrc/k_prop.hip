@@ -230,8 +230,9 @@ void pro_hole_list(const Pro& p, const int* total, hipStream_t s) {
 // holes still empty next to F_r.  The hole count at round r is the count at
 // the start less the frontiers filled, and a count that does not shrink is an
 // empty frontier.  The frontiers alternate between the two halves of the
-// frame's `keys` buffer (2 npix ints) after the hole sub-lists are read; a
-// bit per pixel in the `border` buffer marks holes already queued.
+// frame's `keys` buffer (2 npix ints) after the hole sub-lists are read,
+// each frontier as one list per stage of a round; a bit per pixel in the
+// `border` buffer marks holes already queued.
 constexpr int kFillWaves = 16;
 constexpr int kFillThreads = 64 * kFillWaves;
 
@@ -302,8 +303,10 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
     }
   }
   __syncthreads();
-  // every queued bit is a first-frontier hole: collect them into list 0 from
-  // the bitmap (the sub-lists are no longer needed)
+  // every queued bit is a first-frontier hole: collect them from the bitmap
+  // into a flat list in the second half of the buffer (the sub-lists are no
+  // longer needed)
+  int* const flat = lists + npix;
   {
     const int nwords = (npix + 31) / 32;
     for (int w0 = 0; w0 < nwords; w0 += kFillThreads) {
@@ -324,45 +327,63 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
       while (bits) {
         const int bt = __ffs(bits) - 1;
         bits &= bits - 1;
-        lists[pos++] = w * 32 + bt;
+        flat[pos++] = w * 32 + bt;
       }
     }
   }
-  __syncthreads();
   // Each round runs in three stages: the phases (0,0); (0,1) with (1,0),
   // whose cells are never 4-neighbours of each other, so neither reads the
-  // other's labels; (1,1).  A cell filled in a stage appends its empty,
-  // unqueued 4-neighbours to the next frontier (a hole of this round's
-  // frontier is queued already; a 4-neighbour is never filled in the same
-  // stage).  The frontier counters alternate by round parity: round r reads
-  // fcnt[r & 1] and appends to the other, which round r - 1 read and cleared
-  // after its first stage.
-  __shared__ int fcnt[2];
-  if (tid == 0) {
-    fcnt[0] = ncur;
-    fcnt[1] = 0;
+  // other's labels; (1,1).  A frontier is kept as one list per stage: a
+  // buffer half holds the stage lists at offsets 0, q0 and q0 + q1, where
+  // q0, q1 and npix - q0 - q1 are the frame's cell counts of each stage's
+  // phases, so no list can overflow.  A cell filled in a stage appends its
+  // empty, unqueued 4-neighbours to the next frontier's list of their stage
+  // (a hole of this round's frontier is queued already; a 4-neighbour is
+  // never filled in the same stage): neighbours of (0,0) and (1,1) cells are
+  // all of stage 1, those of a stage-1 cell of stage 0 or 2 by direction.
+  // The counters alternate by round parity: round r reads fcnt[r & 1][..]
+  // and appends to the other set, which round r - 1 read and cleared after
+  // its first stage.
+  const int hx = (W + 1) >> 1, lx = W >> 1, hy = (H + 1) >> 1, ly = H >> 1;
+  const int q0 = hx * hy, q1 = hx * ly + lx * hy;
+  const int qoff[3] = {0, q0, q0 + q1};
+  __shared__ int fcnt[2][3];
+  if (tid < 6) fcnt[tid / 3][tid % 3] = 0;
+  __syncthreads();
+  // the first frontier into the stage lists of the first half
+  for (int i0 = 0; i0 < ncur; i0 += kFillThreads) {
+    const int i = i0 + tid;
+    const int idx = i < ncur ? flat[i] : 0;
+    const int y = idx / W, x = idx - y * W;
+    const int par = ((x & 1) << 1) | (y & 1);
+    const int st = par == 0 ? 0 : par == 3 ? 2 : 1;
+#pragma unroll
+    for (int t = 0; t < 3; t++) wave_append(lists + qoff[t], &fcnt[0][t], i < ncur && st == t, idx);
   }
   __syncthreads();
   int holes = holes0, rounds = 0, stalled = 0;
   int* cur = lists;
   int* nxt = lists + npix;
   for (int r = 0; r < kFillMax; r++) {
-    const int n = fcnt[r & 1];  // block-uniform
+    // this round's counts, read before the first stage's barrier (the
+    // counters are cleared after it)
+    const int nr[3] = {fcnt[r & 1][0], fcnt[r & 1][1], fcnt[r & 1][2]};
+    const int n = nr[0] + nr[1] + nr[2];  // block-uniform
     rounds = r + 1;
     if (holes == 0) break;
     if (n == 0) {  // the count would not shrink
       stalled = 1;
       break;
     }
-    int* const cnt_next = &fcnt[(r + 1) & 1];
+    int* const cnt_next = fcnt[(r + 1) & 1];
     for (int stage = 0; stage < 3; stage++) {
-      for (int i0 = 0; i0 < n; i0 += kFillThreads) {
+      const int ns = nr[stage];
+      const int* const list = cur + qoff[stage];
+      for (int i0 = 0; i0 < ns; i0 += kFillThreads) {
         const int i = i0 + tid;
-        const int idx = i < n ? cur[i] : 0;
+        const bool mine = i < ns;
+        const int idx = mine ? list[i] : 0;
         const int y = idx / W, x = idx - y * W;
-        const int par = ((x & 1) << 1) | (y & 1);  // (xm, ym)
-        const bool mine = i < n && (stage == 0 ? par == 0 : stage == 1 ? (par == 1 || par == 2)
-                                                                      : par == 3);
         // N, S, W, E
         const int nbv[4] = {mine && y > 0 ? seg[idx - W] : -1, mine && y < H - 1 ? seg[idx + W] : -1,
                             mine && x > 0 ? seg[idx - 1] : -1, mine && x < W - 1 ? seg[idx + 1] : -1};
@@ -388,9 +409,12 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
           }
           seg[idx] = lab;
         }
-        // empty 4-neighbours not yet queued join the next frontier
+        // empty 4-neighbours not yet queued join the next frontier, in the
+        // list of their stage: N and S neighbours flip the row parity, W and
+        // E the column parity
         const int nbi[4] = {idx - W, idx + W, idx - 1, idx + 1};
         const bool ok[4] = {mine && y > 0, mine && y < H - 1, mine && x > 0, mine && x < W - 1};
+        const bool ycell = (y & 1) != 0;  // stage 1: a (0,1) cell; else (1,0)
 #pragma unroll
         for (int d = 0; d < 4; d++) {
           bool take = false;
@@ -398,13 +422,21 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
             const uint32_t bit = 1u << (nbi[d] & 31);
             take = !(atomicOr(&queued[nbi[d] >> 5], bit) & bit);
           }
-          wave_append(nxt, cnt_next, take, nbi[d]);
+          if (stage != 1) {  // block-uniform
+            wave_append(nxt + qoff[1], &cnt_next[1], take, nbi[d]);
+          } else {
+            // from (0,1): N/S reach (0,0), W/E reach (1,1); from (1,0) the
+            // other way round
+            const bool to0 = (d < 2) == ycell;
+            wave_append(nxt + qoff[0], &cnt_next[0], take && to0, nbi[d]);
+            wave_append(nxt + qoff[2], &cnt_next[2], take && !to0, nbi[d]);
+          }
         }
       }
       __syncthreads();  // the next stage reads this one's labels
-      // every thread read this round's count before the first barrier, and
-      // the next round appends to it only after two more barriers
-      if (stage == 0 && tid == 0) fcnt[r & 1] = 0;
+      // every thread read this round's counts before the first barrier, and
+      // the next round appends to them only after two more barriers
+      if (stage == 0 && tid < 3) fcnt[r & 1][tid] = 0;
     }
     holes -= n;
     int* t = cur;
