@@ -415,13 +415,15 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
         const int nbi[4] = {idx - W, idx + W, idx - 1, idx + 1};
         const bool ok[4] = {mine && y > 0, mine && y < H - 1, mine && x > 0, mine && x < W - 1};
         const bool ycell = (y & 1) != 0;  // stage 1: a (0,1) cell; else (1,0)
+        // the four queue claims go out together, then the appends
+        uint32_t prev[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+          prev[d] = (ok[d] && nbv[d] == -1) ? atomicOr(&queued[nbi[d] >> 5], 1u << (nbi[d] & 31))
+                                            : 0xffffffffu;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-          bool take = false;
-          if (ok[d] && nbv[d] == -1) {
-            const uint32_t bit = 1u << (nbi[d] & 31);
-            take = !(atomicOr(&queued[nbi[d] >> 5], bit) & bit);
-          }
+          const bool take = !((prev[d] >> (nbi[d] & 31)) & 1u);
           if (stage != 1) {  // block-uniform
             wave_append(nxt + qoff[1], &cnt_next[1], take, nbi[d]);
           } else {
