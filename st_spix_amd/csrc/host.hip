@@ -155,6 +155,14 @@ struct stsp_batch {
   int* h_pro_off = nullptr;
   int* h_fillres = nullptr;
   int* h_negs = nullptr;
+  // batched epilogue: per-frame table, label/pixel block offsets (maxf + 1
+  // each) and the new-label counts
+  EpiFrame* d_epi = nullptr;
+  int* d_epi_off = nullptr;
+  int* d_epi_counts = nullptr;
+  EpiFrame* h_epi = nullptr;
+  int* h_epi_off = nullptr;
+  int* h_epi_counts = nullptr;
 };
 
 struct stsp_ctx {
@@ -396,6 +404,13 @@ static int batch_alloc(stsp_batch* bh, int maxf) {
                         hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&bh->h_negs, sizeof(int) * maxf * kNegSub,
                         hipHostMallocDefault));
+  HIP_TRY(hipMalloc((void**)&bh->d_epi, sizeof(EpiFrame) * maxf));
+  HIP_TRY(hipMalloc((void**)&bh->d_epi_off, sizeof(int) * 2 * (maxf + 1)));
+  HIP_TRY(hipMalloc((void**)&bh->d_epi_counts, sizeof(int) * maxf));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_epi, sizeof(EpiFrame) * maxf, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_epi_off, sizeof(int) * 2 * (maxf + 1),
+                        hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&bh->h_epi_counts, sizeof(int) * maxf, hipHostMallocDefault));
   return STSP_OK;
 }
 
@@ -480,6 +495,15 @@ static void batch_free(stsp_batch* bh) {
   if (bh->h_pro_off) (void)hipHostFree(bh->h_pro_off);
   if (bh->h_fillres) (void)hipHostFree(bh->h_fillres);
   if (bh->h_negs) (void)hipHostFree(bh->h_negs);
+  if (bh->d_epi) (void)hipFree(bh->d_epi);
+  if (bh->d_epi_off) (void)hipFree(bh->d_epi_off);
+  if (bh->d_epi_counts) (void)hipFree(bh->d_epi_counts);
+  if (bh->h_epi) (void)hipHostFree(bh->h_epi);
+  if (bh->h_epi_off) (void)hipHostFree(bh->h_epi_off);
+  if (bh->h_epi_counts) (void)hipHostFree(bh->h_epi_counts);
+  bh->d_epi = nullptr;
+  bh->h_epi = nullptr;
+  bh->d_epi_off = bh->d_epi_counts = bh->h_epi_off = bh->h_epi_counts = nullptr;
   bh->d_pro = nullptr;
   bh->d_pro_off = nullptr;
   bh->d_fillres = bh->d_negs = bh->d_hlen = nullptr;
@@ -1103,26 +1127,6 @@ static int epilogue_finish(stsp_ctx* c, int* spix_out, stsp_params* out, int* ns
   return STSP_OK;
 }
 
-// The batched form of the export: the superpixel count stays on the device
-// (total = nspix_prev + the compaction's new-label count), so no host wait
-// sits between compaction and export; the count and the capacity check are
-// read once after the whole batch.
-__global__ void k_total_from(int* total, const int* num_new, int prev) {
-  *total = prev + *num_new;
-}
-static int epilogue_finish_async(stsp_ctx* c, int* spix_out, const stsp_params* out,
-                                 hipStream_t s) {
-  k_total_from<<<1, 1, 0, s>>>(c->total, c->num_new, c->nspix_prev);
-  launch_fill(c->alive, 0, c->nbuf, s);
-  launch_mark_alive(c->seg, c->npix, c->alive, c->nbuf, s);
-  OutParams o;
-  memcpy(&o, out, sizeof(o));
-  launch_export(c->P, o, c->total, c->alive, out->cap, s);
-  HIP_TRY(hipMemcpyAsync(spix_out, c->seg, sizeof(int) * c->npix, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipGetLastError());
-  return STSP_OK;
-}
-
 static int run_single(stsp_ctx* c, int* spix_out, stsp_params* out, int* nspix_out,
                       hipStream_t s) {
   stsp_ctx* one = c;
@@ -1500,6 +1504,56 @@ int stsp_batch_use_graph(stsp_batch* bh, int on) {
 
 }  // extern "C"
 
+// Compaction and export of every frame of a batch (epilogue_compact and
+// epilogue_finish_async for all frames, one launch per stage): one table
+// upload, the launches, and one copy of the new-label counts, which the
+// caller reads after its wait.
+static int epilogue_batch(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_out,
+                          const stsp_params* outs, hipStream_t s) {
+  int* off_lab = bh->h_epi_off;
+  int* off_pix = bh->h_epi_off + (bh->maxf + 1);
+  int lb = 0, pb = 0;
+  for (int i = 0; i < n; i++) {
+    stsp_ctx* c = ctxs[i];
+    EpiFrame& E = bh->h_epi[i];
+    E.P = c->P;
+    memcpy(&E.o, &outs[i], sizeof(E.o));
+    E.seg = c->seg;
+    E.spix_out = spix_out[i];
+    E.alive = c->alive;
+    E.flags = c->flags;
+    E.rank = c->rank;
+    E.num_new = c->num_new;
+    E.total = c->total;
+    E.ctmp = c->ctmp;
+    E.npix = c->npix;
+    E.nbuf = c->nbuf;
+    E.prev = c->nspix_prev;
+    off_lab[i] = lb;
+    off_pix[i] = pb;
+    lb += div_up(c->nbuf, kBlock);
+    pb += div_up(c->npix, kBlock);
+  }
+  off_lab[n] = lb;
+  off_pix[n] = pb;
+  HIP_TRY(hipMemcpyAsync(bh->d_epi, bh->h_epi, sizeof(EpiFrame) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(bh->d_epi_off, bh->h_epi_off, sizeof(int) * 2 * (bh->maxf + 1),
+                         hipMemcpyHostToDevice, s));
+  EpiBatch e;
+  e.fr = bh->d_epi;
+  e.off_lab = bh->d_epi_off;
+  e.off_pix = bh->d_epi_off + (bh->maxf + 1);
+  e.counts = bh->d_epi_counts;
+  e.nf = n;
+  e.lab_blocks = lb;
+  e.pix_blocks = pb;
+  epi_launch(e, s);
+  HIP_TRY(hipMemcpyAsync(bh->h_epi_counts, bh->d_epi_counts, sizeof(int) * n,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipGetLastError());
+  return STSP_OK;
+}
+
 // Refines frames whose prologues are done: frames with the same schedule
 // (iteration count, mode, sweeps per iteration, split/merge period and start)
 // share one launch sequence; then compaction, one wait, and the exports.
@@ -1531,15 +1585,7 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
     base += (int)group.size();
     slot++;
   }
-  Lanes* L = &bh->lanes;
-  int rc = lanes_fork(L, s, n);
-  if (rc) return rc;
-  for (int i = 0; i < n; i++) {
-    rc = epilogue_compact(ctxs[i], lane_of(L, s, i));
-    if (!rc) rc = epilogue_finish_async(ctxs[i], spix_out[i], &outs[i], lane_of(L, s, i));
-    if (rc) return rc;
-  }
-  rc = lanes_join(L, s);
+  int rc = epilogue_batch(bh, ctxs, n, spix_out, outs, s);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s));  // the counts (and the refinement's timing events)
   for (int i = 0; i < n; i++) tflush(ctxs[i]);  // timed launches are recorded on group leads
@@ -1548,6 +1594,7 @@ static int refine_and_finish(stsp_batch* bh, stsp_ctx** ctxs, int n, int** spix_
     if (rc) return rc;
   }
   for (int i = 0; i < n; i++) {
+    ctxs[i]->host[1] = bh->h_epi_counts[i];
     nspix_out[i] = ctxs[i]->nspix_prev + ctxs[i]->host[1];
     if (nspix_out[i] > outs[i].cap) return STSP_ERR_CAPACITY;
   }

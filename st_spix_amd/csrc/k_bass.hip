@@ -204,10 +204,7 @@ struct CompactTmp {
   float prior_count;
 };
 
-__global__ void k_compact_gather(Params P, const int* flags, const int* rank, int n, int prev,
-                                 CompactTmp* tmp) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || !flags[k]) return;
+__device__ __forceinline__ void compact_gather_one(const Params& P, int k, CompactTmp* dst) {
   CompactTmp t;
   for (int c = 0; c < 3; c++) {
     t.mu_app[c] = P.mu_app[3 * k + c];
@@ -224,15 +221,10 @@ __global__ void k_compact_gather(Params P, const int* flags, const int* rank, in
   t.count = P.count[k];
   t.valid = P.valid[k];
   t.prior_count = P.prior_count[k];
-  tmp[rank[k]] = t;
+  *dst = t;
 }
 
-__global__ void k_compact_scatter(Params P, const CompactTmp* tmp, const int* num_new, int prev,
-                                  int nbuf) {
-  int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= *num_new || prev + d >= nbuf) return;
-  int k = prev + d;
-  const CompactTmp t = tmp[d];
+__device__ __forceinline__ void compact_scatter_one(const Params& P, int k, const CompactTmp& t) {
   for (int c = 0; c < 3; c++) {
     P.mu_app[3 * k + c] = t.mu_app[c];
     P.prior_mu_app[3 * k + c] = t.prior_mu_app[c];
@@ -248,6 +240,20 @@ __global__ void k_compact_scatter(Params P, const CompactTmp* tmp, const int* nu
   P.count[k] = t.count;
   P.valid[k] = t.valid;
   P.prior_count[k] = t.prior_count;
+}
+
+__global__ void k_compact_gather(Params P, const int* flags, const int* rank, int n, int prev,
+                                 CompactTmp* tmp) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !flags[k]) return;
+  compact_gather_one(P, k, tmp + rank[k]);
+}
+
+__global__ void k_compact_scatter(Params P, const CompactTmp* tmp, const int* num_new, int prev,
+                                  int nbuf) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= *num_new || prev + d >= nbuf) return;
+  compact_scatter_one(P, prev + d, tmp[d]);
 }
 
 void launch_compact(int* seg, int npix, Params* P, int n, int prev, int nbuf, int* alive,
@@ -273,11 +279,8 @@ size_t compact_tmp_bytes(int n) { return sizeof(CompactTmp) * (size_t)(n > 0 ? n
 // parameter export (read_params, sparams_io.cu:189-286) and update_prior with
 // invert=false (sparams_io.cu:583-639, rule R10)
 // ---------------------------------------------------------------------------
-__global__ void k_export(Params P, OutParams o, const int* nspix_ptr, const int* alive) {
-  int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  int nspix = *nspix_ptr;
-  if (ix >= nspix || ix >= o.cap) return;
-  bool upd = alive[ix] && P.count[ix] > 0;
+__device__ __forceinline__ void export_one(const Params& P, const OutParams& o, int ix, bool alive) {
+  bool upd = alive && P.count[ix] > 0;
   for (int c = 0; c < 3; c++) {
     o.mu_app[3 * ix + c] = P.mu_app[3 * ix + c];
     o.prior_mu_app[3 * ix + c] = upd ? P.mu_app[3 * ix + c] : P.prior_mu_app[3 * ix + c];
@@ -298,9 +301,125 @@ __global__ void k_export(Params P, OutParams o, const int* nspix_ptr, const int*
   o.ids[ix] = ix;
 }
 
+__global__ void k_export(Params P, OutParams o, const int* nspix_ptr, const int* alive) {
+  int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  int nspix = *nspix_ptr;
+  if (ix >= nspix || ix >= o.cap) return;
+  export_one(P, o, ix, alive[ix] != 0);
+}
+
 void launch_export(Params P, OutParams o, const int* nspix, const int* alive, int nmax,
                    hipStream_t s) {
   k_export<<<div_up(nmax, kBlock), kBlock, 0, s>>>(P, o, nspix, alive);
+}
+
+// ---------------------------------------------------------------------------
+// The compaction and export of every frame of a batch, one launch per stage
+// over all frames (the per-frame launch_compact, count, launch_mark_alive,
+// launch_export and label copy, in the same order).  After the remap the
+// live flags need no second pass over the labels: a label below prev keeps
+// its pixels, and the new labels are exactly prev .. prev + num_new - 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int epi_frame(const int* __restrict__ off, int nf) {
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+#define EPI_LAB                                                     \
+  const int f = epi_frame(e.off_lab, e.nf);                         \
+  const EpiFrame& E = e.fr[f];                                      \
+  const int k = (blockIdx.x - e.off_lab[f]) * kBlock + threadIdx.x; \
+  if (k >= E.nbuf) return;
+#define EPI_PIX                                                     \
+  const int f = epi_frame(e.off_pix, e.nf);                         \
+  const EpiFrame& E = e.fr[f];                                      \
+  const int t = (blockIdx.x - e.off_pix[f]) * kBlock + threadIdx.x; \
+  if (t >= E.npix) return;
+
+__global__ void ke_clear(EpiBatch e) {
+  EPI_LAB
+  E.alive[k] = 0;
+}
+__global__ void ke_mark(EpiBatch e) {
+  EPI_PIX
+  const int v = E.seg[t];
+  if (v >= 0 && v < E.nbuf) E.alive[v] = 1;
+}
+__global__ void ke_flags(EpiBatch e) {
+  EPI_LAB
+  E.flags[k] = (k >= E.prev && E.alive[k]) ? 1 : 0;
+}
+// exclusive scan of each frame's flags (one workgroup per frame, as k_scan1)
+__global__ void __launch_bounds__(1024) ke_scan(EpiBatch e) {
+  __shared__ int part[1024];
+  const EpiFrame& E = e.fr[blockIdx.x];
+  const int n = E.nbuf, tid = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b = tid * chunk, en = min(n, b + chunk);
+  int sum = 0;
+  for (int i = b; i < en; i++) sum += E.flags[i];
+  part[tid] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int i = b; i < en; i++) {
+    const int v = E.flags[i];
+    E.rank[i] = run;
+    run += v;
+  }
+  if (tid == 1023) {
+    const int tot = part[1023];
+    *E.num_new = tot;
+    *E.total = E.prev + tot;
+    e.counts[blockIdx.x] = tot;
+  }
+}
+__global__ void ke_gather(EpiBatch e) {
+  EPI_LAB
+  if (E.flags[k]) compact_gather_one(E.P, k, (CompactTmp*)E.ctmp + E.rank[k]);
+}
+// the remapped labels, also into the caller's label map
+__global__ void ke_remap(EpiBatch e) {
+  EPI_PIX
+  int v = E.seg[t];
+  if (v >= E.prev) {
+    v = E.prev + E.rank[v];
+    E.seg[t] = v;
+  }
+  E.spix_out[t] = v;
+}
+__global__ void ke_scatter(EpiBatch e) {
+  EPI_LAB
+  const int n_new = *E.num_new;
+  if (k < n_new && E.prev + k < E.nbuf) compact_scatter_one(E.P, E.prev + k, ((const CompactTmp*)E.ctmp)[k]);
+  if (k >= E.prev) E.alive[k] = k - E.prev < n_new ? 1 : 0;
+}
+__global__ void ke_export(EpiBatch e) {
+  EPI_LAB
+  if (k >= *E.total || k >= E.o.cap) return;
+  export_one(E.P, E.o, k, E.alive[k] != 0);
+}
+#undef EPI_LAB
+#undef EPI_PIX
+
+void epi_launch(const EpiBatch& e, hipStream_t s) {
+  ke_clear<<<e.lab_blocks, kBlock, 0, s>>>(e);
+  ke_mark<<<e.pix_blocks, kBlock, 0, s>>>(e);
+  ke_flags<<<e.lab_blocks, kBlock, 0, s>>>(e);
+  ke_scan<<<e.nf, 1024, 0, s>>>(e);
+  ke_gather<<<e.lab_blocks, kBlock, 0, s>>>(e);
+  ke_remap<<<e.pix_blocks, kBlock, 0, s>>>(e);
+  ke_scatter<<<e.lab_blocks, kBlock, 0, s>>>(e);
+  ke_export<<<e.lab_blocks, kBlock, 0, s>>>(e);
 }
 
 }  // namespace stsp

@@ -48,6 +48,27 @@ size_t compact_tmp_bytes(int n);
 void launch_export(Params P, OutParams o, const int* nspix, const int* alive, int nmax,
                    hipStream_t s);
 
+// the compaction and export of every frame of a batch (one launch per stage);
+// frame f owns label blocks [off_lab[f], off_lab[f + 1]) and pixel blocks
+// [off_pix[f], off_pix[f + 1]) of kBlock threads
+struct EpiFrame {
+  Params P;
+  OutParams o;
+  int* seg;       // final labels, compacted in place
+  int* spix_out;  // the caller's label map
+  int *alive, *flags, *rank, *num_new, *total;
+  void* ctmp;     // compact_tmp_bytes(nbuf)
+  int npix, nbuf, prev;
+};
+struct EpiBatch {
+  const EpiFrame* fr;
+  const int* off_lab;
+  const int* off_pix;
+  int* counts;  // per frame: the number of new labels
+  int nf, lab_blocks, pix_blocks;
+};
+void epi_launch(const EpiBatch& e, hipStream_t s);
+
 // k_prop.hip: the propagation prologue of every frame of a batch, one
 // launch per stage.  Each frame's pointers and sizes are one ProFrame entry
 // of a device table; ProOff holds, per block kind, the first block of each
