@@ -173,6 +173,69 @@ def alloc_params(cap: int, device, zero: bool = True) -> Params:
     return out
 
 
+_ITEMSIZE = {"float32": 4, "float64": 8, "int32": 4}
+_ALIGN = 256
+
+
+class _OutBatch:
+    """The outputs of a batched call (each frame's label map and parameter
+    arrays of `cap` slots) carved from one device allocation, 256-byte
+    aligned.  The library receives raw pointers; after the call, results()
+    makes every frame's views, the parameter arrays trimmed to the frame's
+    superpixel count, with one split of the buffer."""
+
+    def __init__(self, frames, device):
+        # frames: list of (cap, H, W)
+        self.frames = frames
+        self.fields = []  # per frame: [(offset, aligned bytes)] per field, then spix
+        total = 0
+        for cap, H, W in frames:
+            f = []
+            for _, dt, w in _lib.PARAM_FIELDS:
+                nb = cap * w * _ITEMSIZE[dt]
+                ab = (nb + _ALIGN - 1) // _ALIGN * _ALIGN
+                f.append((total, ab))
+                total += ab
+            ab = (4 * H * W + _ALIGN - 1) // _ALIGN * _ALIGN
+            f.append((total, ab))
+            total += ab
+            self.fields.append(f)
+        self.total = total
+        self.buf = torch.empty(max(total, 1), dtype=torch.uint8, device=device)
+        self.base = self.buf.data_ptr()
+
+    def spix_ptr(self, i: int) -> int:
+        return self.base + self.fields[i][-1][0]
+
+    def params_struct(self, i: int) -> _lib.ParamsStruct:
+        kw = {name: self.base + off
+              for (name, _, _), (off, _) in zip(_lib.PARAM_FIELDS, self.fields[i])}
+        return _lib.ParamsStruct(cap=int(self.frames[i][0]), **kw)
+
+    def results(self, ns):
+        """[(spix, params)] with params trimmed to ns[i] slots."""
+        sizes = []
+        for (cap, H, W), f, n in zip(self.frames, self.fields, ns):
+            for (_, dt, w), (_, ab) in zip(_lib.PARAM_FIELDS, f):
+                used = n * w * _ITEMSIZE[dt]
+                sizes += [used, ab - used]
+            used = 4 * H * W
+            sizes += [used, f[-1][1] - used]
+        parts = torch.split(self.buf[:self.total], sizes)
+        out = []
+        j = 0
+        for (cap, H, W), n in zip(self.frames, ns):
+            p = ParamSet()
+            for name, dt, w in _lib.PARAM_FIELDS:
+                v = parts[j].view(_DTYPES[dt])
+                p[name] = v.view(n, w) if w > 1 else v
+                j += 2
+            spix = parts[j].view(torch.int32).view(H, W)
+            j += 2
+            out.append((spix, p))
+        return out
+
+
 def params_struct(p: Params, cap: Optional[int] = None) -> _lib.ParamsStruct:
     kw = {}
     n = None
@@ -420,7 +483,7 @@ class BatchRunner:
         ctxs, imgs, souts = (VP * n)(), (VP * n)(), (VP * n)()
         outs = (_lib.ParamsStruct * n)()
         nout = (ctypes.c_int * n)()
-        keep = []
+        frames = []
         for i, (ctx, img) in enumerate(items):
             _check_input(img, "img")
             if img.dtype != torch.float32 or img.dim() != 3 or img.shape[2] != 3:
@@ -428,20 +491,20 @@ class BatchRunner:
             if tuple(img.shape[:2]) != (ctx.H, ctx.W):
                 raise RuntimeError("img size does not match its context")
             cap = _lib.lib().stsp_bass_capacity(ctx.cfg.sp_size, ctx.H, ctx.W)
-            out = alloc_params(cap, img.device, zero=False)
-            spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
+            frames.append((cap, ctx.H, ctx.W))
             ctxs[i] = ctx.handle.value
             imgs[i] = img.data_ptr()
-            souts[i] = spix.data_ptr()
-            outs[i] = params_struct(out, cap)
-            keep.append((spix, out))
+        ob = _OutBatch(frames, items[0][1].device)
+        for i in range(n):
+            souts[i] = ob.spix_ptr(i)
+            outs[i] = ob.params_struct(i)
         s = stream if stream is not None else torch.cuda.current_stream()
         key = self._tuned("bass", [it[0] for it in items])
         rc = _lib.lib().stsp_bass_batch(self.handle, ctxs, n, imgs, souts, outs, nout,
                                         ctypes.c_void_p(s.cuda_stream))
         _lib.check(rc, "stsp_bass_batch")
         self._probe(key, [it[0] for it in items], s)
-        return [(keep[i][0], trim(keep[i][1], nout[i])) for i in range(n)]
+        return ob.results([nout[i] for i in range(n)])
 
     def prop(self, items, stream=None):
         """items: list of (ctx, img, flow, spix_prev, params_prev).  Returns a
@@ -462,26 +525,27 @@ class BatchRunner:
         nprev = (ctypes.c_int * n)()
         nout = (ctypes.c_int * n)()
         niters = (ctypes.c_int * n)()
-        keep = []
+        frames = []
         for i, (ctx, img, flow, spix_prev, params_prev) in enumerate(items):
             check_prop_inputs(ctx, img, flow, spix_prev)
             np_ = int(params_prev["ids"].shape[0])
             cap = _lib.lib().stsp_prop_capacity(np_)
-            out = alloc_params(cap, img.device, zero=False)
-            spix = torch.empty((ctx.H, ctx.W), dtype=torch.int32, device=img.device)
+            frames.append((cap, ctx.H, ctx.W))
             ctxs[i] = ctx.handle.value
             imgs[i] = img.data_ptr()
             flows[i] = flow.data_ptr()
             sprev[i] = spix_prev.data_ptr()
-            souts[i] = spix.data_ptr()
             prevs[i] = params_struct(params_prev, np_)
-            outs[i] = params_struct(out, cap)
             nprev[i] = np_
-            keep.append((spix, out))
+        ob = _OutBatch(frames, items[0][1].device)
+        for i in range(n):
+            souts[i] = ob.spix_ptr(i)
+            outs[i] = ob.params_struct(i)
         s = stream if stream is not None else torch.cuda.current_stream()
         key = self._tuned("prop", [it[0] for it in items])
         rc = _lib.lib().stsp_prop_batch(self.handle, ctxs, n, imgs, flows, sprev, prevs, nprev,
                                         souts, outs, nout, niters, ctypes.c_void_p(s.cuda_stream))
         _lib.check(rc, "stsp_prop_batch")
         self._probe(key, [it[0] for it in items], s)
-        return [(keep[i][0], trim(keep[i][1], nout[i]), niters[i]) for i in range(n)]
+        res = ob.results([nout[i] for i in range(n)])
+        return [(res[i][0], res[i][1], niters[i]) for i in range(n)]
