@@ -476,36 +476,58 @@ __global__ void kb_sm_clear(Batch b) {
 __global__ void kb_merge_cand(Batch b, int cur, int direction) {
   PIX_PROLOGUE
   const int W = F.W, H = F.H;
-  const int* seg = F.seg[cur];
-  // (x, y) of pixel t0, then stepped by kBlock pixels (no division per pixel)
-  int y = t0 / W, x = t0 - y * W;
+  const int* __restrict__ seg = F.seg[cur];
+  // (x, y) of pixel t0, then stepped by kBlock pixels (no division per
+  // pixel); the labels of all four pixels and their neighbours are loaded
+  // before any is used
+  int xs[kPixPerThread], ys[kPixPerThread], C[kPixPerThread], Wn[kPixPerThread];
+  bool border[kPixPerThread];
+  {
+    int y = t0 / W, x = t0 - y * W;
+#pragma unroll
+    for (int j = 0; j < kPixPerThread; j++) {
+      if (j > 0) {
+        x += kBlock;
+        while (x >= W) {
+          x -= W;
+          y++;
+        }
+      }
+      xs[j] = x;
+      ys[j] = y;
+    }
+  }
+#pragma unroll
   for (int j = 0; j < kPixPerThread; j++) {
-    if (j > 0) {
-      x += kBlock;
-      while (x >= W) {
-        x -= W;
-        y++;
+    const int t = t0 + j * kBlock, x = xs[j], y = ys[j];
+    const bool valid = t < F.npix;
+    const int c = valid ? seg[t] : -1;
+    const int N = valid && y > 0 ? seg[t - W] : c, S = valid && y < H - 1 ? seg[t + W] : c;
+    const int Wl = valid && x > 0 ? seg[t - 1] : c, E = valid && x < W - 1 ? seg[t + 1] : c;
+    const bool inner = y >= 1 && x >= 1 && y < H - 1 && x < W - 1 && c == N && c == S &&
+                       c == E && c == Wl;
+    int wn = -1;
+    if (valid && !inner) {
+      if (direction == 1) {
+        const bool ok = F.prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
+        if (ok) wn = S;
+      } else {
+        const bool ok = F.prop_mode ? (x > 1 && x < W - 2) : (x > 0 && x < W);
+        if (ok) wn = Wl;
       }
     }
-    const int t = t0 + j * kBlock;
-    if (t >= F.npix) break;
-    const int C = seg[t];
-    const int N = y > 0 ? seg[t - W] : C, S = y < H - 1 ? seg[t + W] : C;
-    const int Wl = x > 0 ? seg[t - 1] : C, E = x < W - 1 ? seg[t + 1] : C;
-    const bool inner = y >= 1 && x >= 1 && y < H - 1 && x < W - 1 && C == N && C == S &&
-                       C == E && C == Wl;
-    if (inner) continue;
-    int Wn = -1;
-    if (direction == 1) {
-      const bool ok = F.prop_mode ? (y > 1 && y < H - 2) : (y >= 0 && y < H - 1);
-      if (ok) Wn = S;
-    } else {
-      const bool ok = F.prop_mode ? (x > 1 && x < W - 2) : (x > 0 && x < W);
-      if (ok) Wn = Wl;
-    }
-    if (Wn >= 0 && C != Wn && !(F.prop_mode && F.P.prop[C] == F.P.prop[Wn]))
-      atomicMax(F.sm.pairs + 2 * C + 1, Wn);
+    C[j] = c;
+    Wn[j] = wn >= 0 && c != wn ? wn : -1;
+    border[j] = Wn[j] >= 0;
   }
+  // prop mode: only pairs of labels that are not both propagated or both new
+  bool take[kPixPerThread];
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++)
+    take[j] = border[j] && !(F.prop_mode && F.P.prop[C[j]] == F.P.prop[Wn[j]]);
+#pragma unroll
+  for (int j = 0; j < kPixPerThread; j++)
+    if (take[j]) atomicMax(F.sm.pairs + 2 * C[j] + 1, Wn[j]);
 }
 
 __global__ void kb_merge_step0(Batch b) {
