@@ -86,9 +86,11 @@ struct BatchOff {
 };
 
 constexpr int kOffArrays = 4 + kSweepShapes;
-// [2i] length and [2i + 1] cursor of list i; [kWorkMidLight] the length of
-// the block list's part filled from its end
-constexpr int kWorkLists = 3, kWorkMidLight = 2 * kWorkLists, kWorkCounters = kWorkMidLight + 1;
+// [2i] length and [2i + 1] cursor of list i (i < 3); [kWorkMidLight] the
+// length of the block list's part filled from its end; [kWorkEsc] and
+// [kWorkEsc + 1] the length and cursor of list 3, the labels whose window
+// search escaped
+constexpr int kWorkLists = 4, kWorkMidLight = 6, kWorkEsc = 7, kWorkCounters = 9;
 constexpr int kMidHeavy = 200 * 200;  // block-list boxes searched first (pixels of box area)
 constexpr int kStatTile = 32;
 

@@ -218,8 +218,9 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
 // search ends after a round in which no own row grew: the level before that
 // round was already a fixed point.  The 512-thread kernel of the block list
 // holds 8 bands (448 rows) of up to 6 words (384 columns), one band per
-// wave; the 512-thread kernel of larger boxes 16 bands (896 rows) of 8 words
-// (512 columns), two bands per wave.
+// wave; the 1024-thread kernel of larger boxes 16 bands (896 rows) of 8 words
+// (512 columns), one band per wave (four waves per SIMD, so at most 128
+// VGPRs: the mask lives in LDS).
 //
 // A label whose box is larger is searched in a window of that size around
 // seed 1 first.  Only pixels connected to seed 1 can move, so the window's
@@ -231,9 +232,9 @@ __global__ void __launch_bounds__(256) kb_bfs_wave(Batch b, int cur, int directi
 // searched: with LDS bitsets up to kBlkWords words, else by per-pixel
 // distances in global memory over the bounding box of each level's frontier.
 constexpr int kHalo = 4, kOwn = 64 - 2 * kHalo;
-constexpr int kBigRows = 16 * kOwn, kBigCols = 8 * 64;  // 512 threads, 2 chunks of 8 bands
+constexpr int kBigRows = 16 * kOwn, kBigCols = 8 * 64;  // 1024 threads, 16 bands
 static_assert(kSplitWinRows == kBigRows && kSplitWinCols == kBigCols,
-              "the widest search window is the 512-thread band layout's");
+              "the widest search window is the 1024-thread band layout's");
 static_assert(kMidRows == 8 * kOwn && kMidCols == 6 * 64,
               "boxes of the block list fit 8 bands of 6 words");
 constexpr int kBlkWords = 2048;
@@ -247,7 +248,7 @@ struct BandEdge {
   uint32_t esc;                               // seed 1's component leaves the window
   uint32_t same;                              // the seeds lie in one piece
 };
-typedef BandEdge<16, 8, 8> BigEdge;
+typedef BandEdge<16, 8, 16> BigEdge;
 
 // Searches the window (bx0, by0, bw, bh) of label k.  esc_sides marks the
 // window edges beyond which the label's box continues.  Returns true
@@ -368,36 +369,35 @@ __device__ __forceinline__ bool bfs_bands(const int* __restrict__ seg, int W, in
       uint32_t ch1 = 0, ch2 = 0;
 #pragma unroll
       for (int c = 0; c < RB; c++) {
-        uint64_t N1[WD], N2[WD];
-#pragma unroll
-        for (int w = 0; w < WD; w++) {
-          N1[w] = N2[w] = 0;
-          const uint32_t bit = 1u << (c * WD + w);
-          if (!((act1 | act2) & bit)) continue;  // wave-uniform
-          const uint64_t m = STSP_MASK(c, w) & ~(V1[c][w] | V2[c][w]);
-          if (act1 & bit) {  // wave-uniform
-            uint64_t h = (V1[c][w] << 1) | (V1[c][w] >> 1);
-            if (w > 0) h |= V1[c][w - 1] >> 63;
-            if (w + 1 < WD) h |= V1[c][w + 1] << 63;
-            N1[w] = (h | lane_up(V1[c][w], 0ull) | lane_down(V1[c][w], 0ull)) & m;
-          }
-          if (act2 & bit) {
-            uint64_t h = (V2[c][w] << 1) | (V2[c][w] >> 1);
-            if (w > 0) h |= V2[c][w - 1] >> 63;
-            if (w + 1 < WD) h |= V2[c][w + 1] << 63;
-            N2[w] = (h | lane_up(V2[c][w], 0ull) | lane_down(V2[c][w], 0ull)) & m;
-          }
-        }
+        // words in order, each updated as soon as it is computed: the left
+        // neighbour's value before this level is kept in p1 / p2, the right
+        // neighbour is not updated yet
+        uint64_t p1 = 0, p2 = 0;
 #pragma unroll
         for (int w = 0; w < WD; w++) {
           const uint32_t bit = 1u << (c * WD + w);
-          if (!((act1 | act2) & bit)) continue;  // N1 = N2 = 0 there
-          const uint64_t n2 = N2[w] & ~N1[w];  // a tie stays with seed 1
-          V1[c][w] |= N1[w];
-          V2[c][w] |= n2;
-          if (__ballot(N1[w] != 0) != 0) ch1 |= bit;
-          if (__ballot(n2 != 0) != 0) ch2 |= bit;
-          if (__ballot(own && (N1[w] | n2) != 0) != 0) grew = 1;
+          const uint64_t o1 = V1[c][w], o2 = V2[c][w];
+          if ((act1 | act2) & bit) {  // wave-uniform
+            const uint64_t m = STSP_MASK(c, w) & ~(o1 | o2);
+            uint64_t n1 = 0, n2 = 0;
+            if (act1 & bit) {  // wave-uniform
+              uint64_t h = (o1 << 1) | (o1 >> 1) | (p1 >> 63);
+              if (w + 1 < WD) h |= V1[c][w + 1] << 63;
+              n1 = (h | lane_up(o1, 0ull) | lane_down(o1, 0ull)) & m;
+            }
+            if (act2 & bit) {
+              uint64_t h = (o2 << 1) | (o2 >> 1) | (p2 >> 63);
+              if (w + 1 < WD) h |= V2[c][w + 1] << 63;
+              n2 = (h | lane_up(o2, 0ull) | lane_down(o2, 0ull)) & m & ~n1;  // ties stay with seed 1
+            }
+            V1[c][w] = o1 | n1;
+            V2[c][w] = o2 | n2;
+            if (__ballot(n1 != 0) != 0) ch1 |= bit;
+            if (__ballot(n2 != 0) != 0) ch2 |= bit;
+            if (__ballot(own && (n1 | n2) != 0) != 0) grew = 1;
+          }
+          p1 = o1;
+          p2 = o2;
         }
       }
       act1 = spread(ch1);
@@ -857,10 +857,7 @@ __device__ __forceinline__ void bfs_global(const int* __restrict__ seg, int W, i
 // whole-box fallbacks).
 template <int NW, int LIST, int MAXR, int MAXC, class Edge>
 __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int direction, Edge& edge,
-                                                uint64_t* sM, uint64_t* sV1, uint64_t* sV2,
-                                                uint64_t* sNK, int* changed, int (*fb)[4],
-                                                int& item) {
-  constexpr int NT = 64 * NW;
+                                                uint64_t* sM, int& item) {
   const int tid = threadIdx.x;
   const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[2 * LIST], b.work_cap));
   while (true) {
@@ -890,17 +887,12 @@ __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int dir
     if (!(bh <= win_h && bw <= win_w) && bw <= win_h && bh <= win_w) {
       // too wide for the bands, but it fits them transposed: searched whole
       // (box columns as band rows), no window and no fallback
-      static_assert(NW == 8, "the big-box kernel has 8 waves");
-#define STSP_BANDS_T(RB, WD)                                                                  \
-  (void)bfs_bands<NW, RB, WD, true, Edge, true>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, \
-                                                edge, 0, sM)
-      if (bw <= NW * kOwn) {
-        if (bh <= 256) STSP_BANDS_T(1, 4);
-        else STSP_BANDS_T(1, 8);
-      } else {
-        if (bh <= 256) STSP_BANDS_T(2, 4);
-        else STSP_BANDS_T(2, 8);
-      }
+      static_assert(NW * kOwn >= MAXR, "one band per wave covers the window's rows");
+#define STSP_BANDS_T(WD)                                                                     \
+  (void)bfs_bands<NW, 1, WD, true, Edge, true>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, \
+                                               edge, 0, sM)
+      if (bh <= 256) STSP_BANDS_T(4);
+      else STSP_BANDS_T(8);
 #undef STSP_BANDS_T
       if (tid == 0) atomicAdd(F.search_count + kSearchBand, 1ull);
       continue;
@@ -922,29 +914,59 @@ __device__ __forceinline__ void bfs_block_items(const Batch& b, int cur, int dir
       if (y < wy0 || y >= wy0 + wh || x < wx0 || x >= wx0 + ww) t2 = -1;
     }
     bool escaped;
-#define STSP_BANDS(RB, WD)                                                                     \
-  escaped = bfs_bands<NW, RB, WD, (NW > 4)>(seg, W, k, wx0, wy0, ww, wh, s1, t2, k2, halves, edge, \
-                                            sides, sM)
-    const int rb = wh <= NW * kOwn ? 1 : 2;
-    static_assert(NW == 8, "the big-box kernel has 8 waves");
-    {
-      const int wd = ww <= 256 ? 4 : 8;
-      if (rb == 1) {
-        if (wd == 4) STSP_BANDS(1, 4);
-        else STSP_BANDS(1, 8);
-      } else {
-        if (wd == 4) STSP_BANDS(2, 4);
-        else STSP_BANDS(2, 8);
+#define STSP_BANDS(WD)                                                                          \
+  escaped = bfs_bands<NW, 1, WD, true>(seg, W, k, wx0, wy0, ww, wh, s1, t2, k2, halves, edge, sides, \
+                                       sM)
+    if (ww <= 256) STSP_BANDS(4);
+    else STSP_BANDS(8);
+#undef STSP_BANDS
+    if (tid == 0) {
+      atomicAdd(F.search_count + (escaped ? kSearchWhole : sides ? kSearchWindow : kSearchBand), 1ull);
+      // the whole-box search of an escaped window runs in kb_bfs_fallback
+      if (escaped) {
+        const int pos = atomicAdd(b.work_n + kWorkEsc, 1);
+        if (pos < b.work_cap) b.work[3 * b.work_cap + pos] = wv;
       }
     }
-#undef STSP_BANDS
-    if (tid == 0)
-      atomicAdd(F.search_count + (escaped ? kSearchWhole : sides ? kSearchWindow : kSearchBand), 1ull);
-    if (!escaped) continue;
+  }
+}
+
+// The whole-box searches of the labels whose window search escaped (the
+// escape list, work[3 work_cap..)): with LDS bitsets up to kBlkWords words,
+// LDS bitsets with the words owned by threads up to kBigWords, else by
+// per-pixel distances in global memory.  NW waves per workgroup.
+template <int NW>
+__device__ __forceinline__ void bfs_fallback_items(const Batch& b, int cur, int direction,
+                                                   uint64_t* sM, uint64_t* sV1, uint64_t* sV2,
+                                                   uint64_t* sNK, int* changed, int (*fb)[4],
+                                                   int& item) {
+  constexpr int NT = 64 * NW;
+  const int tid = threadIdx.x;
+  const int n_items = __builtin_amdgcn_readfirstlane(min(b.work_n[kWorkEsc], b.work_cap));
+  while (true) {
+    __syncthreads();
+    if (tid == 0) item = atomicAdd(b.work_n + kWorkEsc + 1, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(item);
+    if (it >= n_items) break;
+    const int wv = __builtin_amdgcn_readfirstlane(b.work[3 * b.work_cap + it]);
+    const int f = wv >> 24, k = wv & 0xffffff;
+    const FrameDesc& F = b.fr[f];
+    const int* seg = F.seg[cur];
+    const int W = F.W;
+    const int* bb = F.st.bbox + 4 * k;
+    const int bx0 = __builtin_amdgcn_readfirstlane(bb[0]);
+    const int by0 = __builtin_amdgcn_readfirstlane(bb[2]);
+    const int bw = __builtin_amdgcn_readfirstlane(bb[1]) - bx0 + 1;
+    const int bh = __builtin_amdgcn_readfirstlane(bb[3]) - by0 + 1;
+    const int s1 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, direction));
+    const int s2 = __builtin_amdgcn_readfirstlane(seed_of(F, seg, k, -direction));
+    const int k2 = k + __builtin_amdgcn_readfirstlane(*F.max_spix) + 1;
+    int* halves = F.halves;
     const int nwords = bh * ((bw + 63) >> 6);
     if (nwords <= kBlkWords)
       bfs_lds<NT>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, sM, sV1, sV2, sNK, changed);
-    else if (NW > 4 && nwords <= kBigWords)  // sM.. are one pool of 2 kBigWords here
+    else if (nwords <= kBigWords)  // sM.. are one pool of 2 kBigWords here
       bfs_lds_owned<NT, kBigWords / NT>(seg, W, k, bx0, by0, bw, bh, s1, s2, k2, halves, sM,
                                         sM + kBigWords, changed);
     else
@@ -1021,19 +1043,22 @@ __global__ void __launch_bounds__(512) kb_bfs_mid(Batch b, int cur, int directio
   }
 }
 
-// One LDS pool: the band mask (16 bands x 8 words x 64 lanes) and the band
-// edges, or the bitsets of a whole-box search (up to kBigWords words), which
-// runs only after the bands are done.
+// The whole-box searches' LDS: the bitsets of up to kBigWords words (two
+// pools of kBigWords, or four of kBlkWords).
 constexpr int kBigPool = 2 * kBigWords;
-static_assert(16 * 8 * 64 <= 4 * kBlkWords && 4 * kBlkWords * 8 + sizeof(BigEdge) <= kBigPool * 8,
-              "the band mask and edges fit the pool");
-__global__ void __launch_bounds__(512) kb_bfs_big(Batch b, int cur, int direction) {
+static_assert(4 * kBlkWords <= kBigPool, "four bitsets of kBlkWords fit the pool");
+__global__ void __launch_bounds__(1024) kb_bfs_big(Batch b, int cur, int direction) {
+  __shared__ uint64_t pool[16 * 8 * 64];  // the band mask
+  __shared__ BigEdge edge;
+  __shared__ int item;
+  bfs_block_items<16, 2, kBigRows, kBigCols>(b, cur, direction, edge, pool, item);
+}
+
+__global__ void __launch_bounds__(512) kb_bfs_fallback(Batch b, int cur, int direction) {
   __shared__ uint64_t pool[kBigPool];
   __shared__ int item, changed[2], fb[3][4];
-  BigEdge& edge = *reinterpret_cast<BigEdge*>(pool + 4 * kBlkWords);
-  bfs_block_items<8, 2, kBigRows, kBigCols>(b, cur, direction, edge, pool, pool + kBlkWords,
-                                             pool + 2 * kBlkWords, pool + 3 * kBlkWords, changed,
-                                             fb, item);
+  bfs_fallback_items<8>(b, cur, direction, pool, pool + kBlkWords, pool + 2 * kBlkWords,
+                        pool + 3 * kBlkWords, changed, fb, item);
 }
 
 __global__ void kb_split_eval(Batch b) {
@@ -1218,7 +1243,8 @@ void b_split(const Batch& b, int cur, int direction, hipStream_t s, const SideSt
     (void)hipStreamWaitEvent(side.st, side.fork, 0);
     (void)hipStreamWaitEvent(side.st2, side.fork, 0);
   }
-  kb_bfs_big<<<64, 512, 0, side.st ? side.st2 : s>>>(b, cur, direction);
+  kb_bfs_big<<<64, 1024, 0, side.st ? side.st2 : s>>>(b, cur, direction);
+  kb_bfs_fallback<<<16, 512, 0, side.st ? side.st2 : s>>>(b, cur, direction);
   kb_bfs_mid<<<256, 512, 0, side.st ? side.st : s>>>(b, cur, direction);
   kb_bfs_wave<<<2048, 256, 0, s>>>(b, cur, direction);
   if (side.st) {
