@@ -725,11 +725,11 @@ static void iteration_tail(stsp_ctx* lead, const Batch& b, const SideStream& sid
                            int prop_mode, int shape, int* cur, int* smc, hipStream_t s) {
   const stsp_config& cfg = lead->cfg;
   // split and merge leave the label statistics current (kb_split_stats,
-  // kb_merge_stats): only the parameters are updated after them
+  // kb_merge_stats); the split also updates the parameters of the labels it
+  // changed, the merge is followed by a parameter update
   if (is_split_iter(cfg, idx)) {
     *smc += 1;
     b_split(b, *cur, *smc % 2 + 1, s, side);
-    b_param_only(b, s);
   }
   if (is_merge_iter(cfg, idx)) {
     if (prop_mode) {

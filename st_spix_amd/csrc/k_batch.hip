@@ -196,99 +196,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
 __global__ void kb_param_update(Batch b, int summary_mode) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
-  Params P = F.P;
-  LabelStats st = F.st;
-  const int sp_size = F.sp_size;
-  P.mu_app[3 * k] = P.mu_app[3 * k + 1] = P.mu_app[3 * k + 2] = 0.f;
-  P.mu_shape[2 * k] = P.mu_shape[2 * k + 1] = 0.0;
-  P.sigma_shape[3 * k] = P.sigma_shape[3 * k + 1] = P.sigma_shape[3 * k + 2] = 0.0;
-  const int count_int = st.count[k];
-  P.count[k] = count_int;
-  if (P.valid[k] != 0) {
-    if (count_int <= 0) {
-      P.valid[k] = 0;
-    } else if (summary_mode) {
-      double count = count_int * 1.0;
-      for (int c = 0; c < 3; c++)
-        P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
-      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
-      P.mu_shape[2 * k] = mx;
-      P.mu_shape[2 * k + 1] = my;
-      double sx = (double)st.X[5 * k + 2] / count - mx * mx;
-      double sy = (double)st.X[5 * k + 3] / count - mx * my;
-      double sz = (double)st.X[5 * k + 4] / count - my * my;
-      double det = sx * sz - sy * sy;
-      if (det <= 0) {
-        sx = sx + 0.00001;
-        sz = sz + 0.00001;
-        det = sx * sz - sy * sy;
-        if (det <= 0) det = 0.00001;
-      }
-      P.sigma_shape[3 * k] = sx;
-      P.sigma_shape[3 * k + 1] = sy;
-      P.sigma_shape[3 * k + 2] = sz;
-      P.logdet[k] = stsp_det_log(det);
-    } else {
-      float pc = P.prior_count[k];
-      double count = count_int * 1.0;
-      double total_count = (double)count_int + (double)(pc * 50.0f);
-      double pi[3];
-      if (P.prop[k]) {
-        int cnt = (int)pc;  // _add_sigma_smoothing(sample_sigma_shape, (int)pc, pc, sp_size)
-        double tc = 1. * (float)((float)cnt + pc * 50.0f);
-        const double* in = P.sample_sigma_shape + 3 * k;
-        pi[0] = ((double)(pc * (float)sp_size) + cnt * in[0]) / (tc + 3.0);
-        pi[1] = (cnt * in[1]) / (tc + 3.0);
-        pi[2] = ((double)(pc * (float)sp_size) + cnt * in[2]) / (tc + 3.0);
-      } else {
-        pi[0] = P.prior_icov[3 * k];
-        pi[1] = P.prior_icov[3 * k + 1];
-        pi[2] = P.prior_icov[3 * k + 2];
-      }
-      double pr_det = pi[0] * pi[2] - pi[1] * pi[1];
-      pr_det = sqrt(pr_det);
-      double ps0 = (double)pc / pr_det * pi[0];
-      double ps1 = (double)pc / pr_det * pi[1];
-      double ps2 = (double)pc / pr_det * pi[2];
-      for (int c = 0; c < 3; c++)
-        P.mu_app[3 * k + c] = (float)(fx_to_double(st.S[3 * k + c]) / count);
-      double mx = (double)st.X[5 * k] / count, my = (double)st.X[5 * k + 1] / count;
-      P.mu_shape[2 * k] = mx;
-      P.mu_shape[2 * k + 1] = my;
-      double sx = (double)st.X[5 * k + 2] - count * mx * mx;
-      double sy = (double)st.X[5 * k + 3] - count * mx * my;
-      double sz = (double)st.X[5 * k + 4] - count * my * my;
-      sx = ((double)pc * ps0 + sx) / (total_count + 3.0);
-      sy = ((double)pc * ps1 + sy) / (total_count + 3.0);
-      sz = ((double)pc * ps2 + sz) / (total_count + 3.0);
-      double det = sx * sz - sy * sy;
-      if (det <= 0) {
-        sx = sx + 0.00001;
-        sz = sz + 0.00001;
-        det = sx * sz - sy * sy;
-        if (det <= 0) det = 0.00001;
-      }
-      P.sigma_shape[3 * k] = sz / det;
-      P.sigma_shape[3 * k + 1] = -sy / det;
-      P.sigma_shape[3 * k + 2] = sx / det;
-      P.logdet[k] = stsp_det_log(det);
-    }
-  }
-  SweepRec r;
-  r.mu[0] = P.mu_app[3 * k];
-  r.mu[1] = P.mu_app[3 * k + 1];
-  r.mu[2] = P.mu_app[3 * k + 2];
-  r.logdet = (float)P.logdet[k];
-  r.isig[0] = (float)P.sigma_shape[3 * k];
-  r.isig[1] = (float)P.sigma_shape[3 * k + 1];
-  r.isig[2] = (float)P.sigma_shape[3 * k + 2];
-  r.count = P.count[k];
-  const double mx = P.mu_shape[2 * k], my = P.mu_shape[2 * k + 1];
-  r.cx = (int)ceil(mx);
-  r.fx = (int)floor(mx);
-  r.cy = (int)ceil(my);
-  r.fy = (int)floor(my);
-  F.rec[k] = r;
+  param_update_one(F, k, summary_mode);
 }
 
 // store_sample_sigma_shape_k (update_params.cu:396-438)
@@ -439,6 +347,12 @@ __global__ void kb_reset_work(Batch b) {
 // split (split_merge_orig.cu:202-420, split_merge_prop.cu:184-395)
 // ---------------------------------------------------------------------------
 __global__ void kb_sm_clear(Batch b) {
+  // block 0 also clears the work-list counters and each frame's counter
+  // (kb_reset_work)
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < kWorkCounters) b.work_n[threadIdx.x] = 0;
+    if (threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
+  }
   SM_PROLOGUE
   if (k >= F.nsm) return;
   LabelStats sst = F.sst;
@@ -535,7 +449,11 @@ __global__ void kb_merge_step0(Batch b) {
   if (k >= F.nbuf || F.P.valid[k] == 0) return;
   SmState sm = F.sm;
   int fk = sm.pairs[2 * k + 1];
-  if (fk < 0) return;
+  // the marginal of the label alone (b_n stays cleared without a candidate)
+  if (fk < 0) {
+    sm.marg[k] = nig_marg(F.nc, (double)F.P.count[k], sm.b_n + 3 * k);
+    return;
+  }
   const NigConsts& nc = F.nc;
   int count_f = F.P.count[fk], count_k = F.P.count[k];
   int count_fk = count_f + count_k;
@@ -551,12 +469,7 @@ __global__ void kb_merge_step0(Batch b) {
     sm.b_n_f[3 * k + c] = bf;
   }
   sm.has_bn[k] = 1;
-}
-
-__global__ void kb_merge_marg(Batch b) {
-  LAB_PROLOGUE
-  if (k >= F.nbuf || F.P.valid[k] == 0) return;
-  F.sm.marg[k] = nig_marg(F.nc, (double)F.P.count[k], F.sm.b_n + 3 * k);
+  sm.marg[k] = nig_marg(nc, (double)count_k, sm.b_n + 3 * k);
 }
 
 __global__ void kb_merge_eval(Batch b) {
@@ -585,6 +498,18 @@ __global__ void kb_merge_flag(Batch b) {
   sm.valid0[k] = F.P.valid[k];
   sm.pc0[k] = F.P.prior_count[k];
   sm.rank[k] = sm.pairs[2 * k + 1];  // target snapshot
+  // the live statistics are cleared for kb_merge_stats (nothing reads them
+  // between here and there)
+  const LabelStats& st = F.st;
+  st.count[k] = 0;
+  st.ninvalid[k] = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    st.S[3 * k + c] = 0;
+    st.Q[3 * k + c] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) st.X[5 * k + c] = 0;
 }
 
 __global__ void kb_merge_remove(Batch b) {
@@ -686,12 +611,10 @@ void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
   kb_merge_cand<<<b.pix_blocks, kBlock, 0, s>>>(b, cur, direction);
   kb_merge_gather<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_step0<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  kb_merge_marg<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_eval<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_flag<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_remove<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_merge_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
-  kb_stats_clear<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
   kb_merge_stats<<<b.lab_blocks, kBlock, 0, s>>>(b);
 }
 
@@ -864,7 +787,6 @@ __global__ void kb_relabel_max(Batch b) {
 // launches shared with the split step (k_split.hip)
 void b_sm_begin(const Batch& b, hipStream_t s) {
   kb_sm_clear<<<b.sm_blocks, kBlock, 0, s>>>(b);
-  kb_reset_work<<<1, 64, 0, s>>>(b);
 }
 
 void b_moved_stats(const Batch& b, int cur, hipStream_t s) {

@@ -1192,7 +1192,8 @@ __global__ void kb_split_apply(Batch b, int cur) {
 // kept their pixels, so an accepted label takes the sums of the part that
 // stays and its new label those of the part that moves (the assignment of
 // kb_split_apply).  Must run before kb_add_max changes max_spix.  (The
-// invalid counts are not kept for live labels.)
+// invalid counts are not kept for live labels.)  The parameters of the two
+// labels follow.
 __global__ void kb_split_stats(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
@@ -1218,6 +1219,10 @@ __global__ void kb_split_stats(Batch b) {
     st.X[5 * k + c] = h.X[5 * stay + c];
     st.X[5 * s + c] = h.X[5 * go + c];
   }
+  // only the two labels of an accepted split changed statistics or priors
+  // (kb_split_assign, kb_split_apply): their parameters are updated here
+  param_update_one(F, k, 0);
+  param_update_one(F, s, 0);
 }
 
 __global__ void kb_add_max(Batch b) {
