@@ -338,17 +338,12 @@ __global__ void __launch_bounds__(1024) kb_scan(Batch b, int which) {
   if (tid == 0) *F.total = carry;
 }
 
-__global__ void kb_reset_work(Batch b) {
-  if (threadIdx.x < kWorkCounters && blockIdx.x == 0) b.work_n[threadIdx.x] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
-}
 
 // ---------------------------------------------------------------------------
 // split (split_merge_orig.cu:202-420, split_merge_prop.cu:184-395)
 // ---------------------------------------------------------------------------
 __global__ void kb_sm_clear(Batch b) {
   // block 0 also clears the work-list counters and each frame's counter
-  // (kb_reset_work)
   if (blockIdx.x == 0) {
     if (threadIdx.x < kWorkCounters) b.work_n[threadIdx.x] = 0;
     if (threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
@@ -624,6 +619,11 @@ void b_merge(const Batch& b, int cur, int direction, hipStream_t s) {
 // that are dead now, and the label itself; the search scans exactly those.
 // ---------------------------------------------------------------------------
 __global__ void kb_relabel_prep(Batch b) {
+  // block 0 also clears the work-list counters and each frame's counter
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < kWorkCounters) b.work_n[threadIdx.x] = 0;
+    if (threadIdx.x < b.nf) b.fr[threadIdx.x].counter[0] = 0;
+  }
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
   F.alive[k] = 0;
@@ -728,9 +728,18 @@ __global__ void kb_relabel_assign(Batch b) {
   if (t >= 0) atomicMax(F.src_of + t, k);
 }
 
+// also the valid flags: a label with a target gives up its id for it
+// (nothing else in this pass reads them)
 __global__ void kb_relabel_params(Batch b) {
   LAB_PROLOGUE
   if (k >= F.nbuf) return;
+  if (k <= *F.max_spix) {
+    const int t = F.target[k];
+    if (t >= 0) {
+      F.P.valid[k] = 0;
+      F.P.valid[t] = 1;
+    }
+  }
   const int s = F.src_of[k];
   if (s < 0) return;
   Params P = F.P;
@@ -748,17 +757,16 @@ __global__ void kb_relabel_params(Batch b) {
   }
 }
 
-__global__ void kb_relabel_valid(Batch b) {
-  LAB_PROLOGUE
-  if (k > *F.max_spix || k >= F.nbuf) return;
-  int t = F.target[k];
-  if (t >= 0) {
-    F.P.valid[k] = 0;
-    F.P.valid[t] = 1;
-  }
-}
 
 __global__ void kb_relabel_apply(Batch b, int cur) {
+  // block 0 also moves each frame's max_spix past the new ids (no pixel of
+  // this pass reads it)
+  if (blockIdx.x == 0 && threadIdx.x < b.nf) {
+    const FrameDesc& G = b.fr[threadIdx.x];
+    int m = *G.max_spix + *G.total;
+    if (m > G.nbuf - 1) m = G.nbuf - 1 > *G.max_spix ? G.nbuf - 1 : *G.max_spix;
+    *G.max_spix = m;
+  }
   PIX_PROLOGUE
   int* seg = F.seg[cur];
   int L[kPixPerThread];
@@ -775,14 +783,6 @@ __global__ void kb_relabel_apply(Batch b, int cur) {
   }
 }
 
-__global__ void kb_relabel_max(Batch b) {
-  int f = threadIdx.x;
-  if (f >= b.nf) return;
-  const FrameDesc& F = b.fr[f];
-  int m = *F.max_spix + *F.total;
-  if (m > F.nbuf - 1) m = F.nbuf - 1 > *F.max_spix ? F.nbuf - 1 : *F.max_spix;
-  *F.max_spix = m;
-}
 
 // launches shared with the split step (k_split.hip)
 void b_sm_begin(const Batch& b, hipStream_t s) {
@@ -800,15 +800,12 @@ void b_scan(const Batch& b, int which, hipStream_t s) {
 void b_relabel(const Batch& b, int cur, hipStream_t s) {
   kb_relabel_prep<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_mark_alive<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
-  kb_reset_work<<<1, 64, 0, s>>>(b);
   kb_relabel_lists<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_relabel_search<<<512, 256, 0, s>>>(b);
   kb_scan<<<b.nf, 1024, 0, s>>>(b, 1);
   kb_relabel_assign<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_relabel_params<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  kb_relabel_valid<<<b.lab_blocks, kBlock, 0, s>>>(b);
   kb_relabel_apply<<<b.pix_blocks, kBlock, 0, s>>>(b, cur);
-  kb_relabel_max<<<1, 64, 0, s>>>(b);
 }
 
 }  // namespace stsp
