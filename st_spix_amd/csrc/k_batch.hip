@@ -50,12 +50,11 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 // keep them current with the other sums).
 // BBOX: also the bounding boxes, the halves map (a copy of the labels) and,
 // for propagated frames, the count of pixels whose shifted label is invalid
-// (ninvalid): the pass before a split.
-// SUMS = false: only what BBOX and SQ add, when the other sums are already
-// current (the sweeps keep them)
-template <bool BBOX, bool SQ, bool SUMS = true, bool MOVED = false>
+// (ninvalid): a full pass at the first iteration when it splits (the pass
+// before a later split, whose sums are current, is kb_bbox_accum).
+template <bool BBOX, bool SQ, bool MOVED = false>
 __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
-  static_assert(!MOVED || (SUMS && SQ && !BBOX), "the moved parts take every sum");
+  static_assert(!MOVED || (SQ && !BBOX), "the moved parts take every sum");
   const int f = frame_of(b.off.st, b.nf, blockIdx.x);
   const FrameDesc& F = b.fr[f];
   const int W = F.W, H = F.H;
@@ -89,12 +88,10 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
       if (v >= lo && v < nlab) {
         k[j] = v;
         pend |= 1u << j;
-        if (SUMS || SQ) {
-          const float* p = F.img + 3 * t;
-          col[j][0] = p[0];
-          col[j][1] = p[1];
-          col[j][2] = p[2];
-        }
+        const float* p = F.img + 3 * t;
+        col[j][0] = p[0];
+        col[j][1] = p[1];
+        col[j][2] = p[2];
         if (shifted) inv[j] = shifted[t] < 0;
       }
     }
@@ -115,16 +112,14 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
         const int dx = lx + j;
         cnt += 1;
         ninv += inv[j];
-        if (SUMS) {
-          sdx += dx;
-          sdy += ly;
-          sxx += dx * dx;
-          sxy += dx * ly;
-          syy += ly * ly;
-        }
+        sdx += dx;
+        sdy += ly;
+        sxx += dx * dx;
+        sxy += dx * ly;
+        syy += ly * ly;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          if (SUMS) S[c] += fx_color(col[j][c]);
+          S[c] += fx_color(col[j][c]);
           if (SQ) Q[c] += fx_square(col[j][c]);
         }
         if (BBOX) {
@@ -135,17 +130,15 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
     }
     int mny = cnt ? y : 0x7fffffff, mxy = cnt ? y : -1;
     if (shifted) ninv = wave_sum_to63(ninv);
-    if (SUMS) {
-      cnt = wave_sum_to63(cnt);
-      sdx = wave_sum_to63(sdx);
-      sdy = wave_sum_to63(sdy);
-      sxx = wave_sum_to63(sxx);
-      sxy = wave_sum_to63(sxy);
-      syy = wave_sum_to63(syy);
-    }
+    cnt = wave_sum_to63(cnt);
+    sdx = wave_sum_to63(sdx);
+    sdy = wave_sum_to63(sdy);
+    sxx = wave_sum_to63(sxx);
+    sxy = wave_sum_to63(sxy);
+    syy = wave_sum_to63(syy);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      if (SUMS) S[c] = wave_sum64_to63(S[c]);
+      S[c] = wave_sum64_to63(S[c]);
       if (SQ) Q[c] = wave_sum64_to63(Q[c]);
     }
     if (BBOX) {
@@ -154,18 +147,7 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
       mny = wave_min_to63(mny);
       mxy = wave_max_to63(mxy);
     }
-    if (lane == 63 && !SUMS && BBOX) {
-      if (SQ) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) atomic_add_i64(st.Q + 3 * u + c, Q[c]);
-      }
-      if (ninv) atomicAdd(st.ninvalid + u, ninv);
-      atomicMin(st.bbox + 4 * u, mnx);
-      atomicMax(st.bbox + 4 * u + 1, mxx);
-      atomicMin(st.bbox + 4 * u + 2, mny);
-      atomicMax(st.bbox + 4 * u + 3, mxy);
-    }
-    if (lane == 63 && SUMS) {
+    if (lane == 63) {
       const int64_t n = cnt, X0 = tx0, Y0 = ty0;
       atomicAdd(st.count + u, cnt);
       if (ninv) atomicAdd(st.ninvalid + u, ninv);
@@ -219,6 +201,82 @@ __global__ void kb_store_sample(Batch b) {
   }
 }
 
+// The pass before a split: bounding boxes and invalid counts of the live
+// labels, and the halves map (a copy of the labels).  A workgroup covers a
+// kStatTile x kStatTile tile, wave w its rows 8w..8w+7; pixel slot j of a
+// lane is column lane & 31 of row 8w + 2j + (lane >> 5), so every load and
+// store of a slot covers two whole 128-byte row segments.  Per distinct
+// label of a wave the extents come from ballots of its pixels, in scalar
+// registers: the columns from the OR of the slots' half-wave masks, the rows
+// from which half-wave masks are non-empty.
+__global__ void __launch_bounds__(256) kb_bbox_accum(Batch b, int cur) {
+  static_assert(kStatTile == 32, "a half-wave covers one tile row");
+  const int f = frame_of(b.off.st, b.nf, blockIdx.x);
+  const FrameDesc& F = b.fr[f];
+  const int W = F.W, H = F.H;
+  const int tile = blockIdx.x - b.off.st[f];
+  const int tiles_x = (W + kStatTile - 1) / kStatTile;
+  const int tx0 = (tile % tiles_x) * kStatTile, ty0 = (tile / tiles_x) * kStatTile;
+  const int lane = threadIdx.x & 63;
+  const int wrow = 8 * (threadIdx.x >> 6);  // the wave's first row of the tile
+  const int x = tx0 + (lane & 31), y0 = ty0 + wrow + (lane >> 5);
+  const int* labs = F.seg[cur];
+  const int* shifted = F.prop_mode ? F.shifted : nullptr;
+  // all loads of the lane's four pixels go out before the first is used
+  int k[4], sh[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int y = y0 + 2 * j;
+    const bool in = x < W && y < H;
+    const int t = in ? y * W + x : 0;
+    k[j] = in ? labs[t] : -1;
+    sh[j] = (in && shifted) ? shifted[t] : 0;
+  }
+  unsigned pend = 0, invm = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int y = y0 + 2 * j;
+    if (x < W && y < H) F.halves[y * W + x] = k[j];
+    if (k[j] >= 0 && k[j] < F.nbuf) {
+      pend |= 1u << j;
+      if (sh[j] < 0) invm |= 1u << j;
+    } else {
+      k[j] = -1;
+    }
+  }
+  while (true) {
+    const uint64_t act = __ballot(pend != 0);
+    if (act == 0) break;
+    const int leader = __builtin_ctzll(act);
+    const int first = (pend & 1u) ? k[0] : (pend & 2u) ? k[1] : (pend & 4u) ? k[2] : k[3];
+    const int u = __builtin_amdgcn_readlane(first, leader);
+    unsigned mine = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (((pend >> j) & 1u) && k[j] == u) mine |= 1u << j;
+    pend &= ~mine;
+    uint32_t cols = 0, rows = 0;  // rows: bit 2j + h of the wave's 8 rows
+    int ninv = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint64_t m = __ballot((mine >> j) & 1u);
+      const uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+      cols |= lo | hi;
+      rows |= (lo ? 1u : 0u) << (2 * j);
+      rows |= (hi ? 1u : 0u) << (2 * j + 1);
+      if (shifted) ninv += __popcll(__ballot(((mine & invm) >> j) & 1u));
+    }
+    if (lane == 0) {
+      int* bb = F.st.bbox + 4 * u;
+      atomicMin(bb, tx0 + __builtin_ctz(cols));
+      atomicMax(bb + 1, tx0 + 31 - __builtin_clz(cols));
+      atomicMin(bb + 2, ty0 + wrow + __builtin_ctz(rows));
+      atomicMax(bb + 3, ty0 + wrow + 31 - __builtin_clz(rows));
+      if (ninv) atomicAdd(F.st.ninvalid + u, ninv);
+    }
+  }
+}
+
 // parameters from the statistics as they stand (kept current by the sweeps)
 void b_param_only(const Batch& b, hipStream_t s) {
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
@@ -250,7 +308,7 @@ __global__ void kb_bbox_clear(Batch b) {
 // then the parameters are updated.
 void b_prepare_split(const Batch& b, int cur, hipStream_t s) {
   kb_bbox_clear<<<b.lab_blocks, kBlock, 0, s>>>(b);
-  kb_stats_accum<true, false, false><<<b.st_blocks, 256, 0, s>>>(b, cur);
+  kb_bbox_accum<<<b.st_blocks, 256, 0, s>>>(b, cur);
   kb_param_update<<<b.lab_blocks, kBlock, 0, s>>>(b, 0);
 }
 
@@ -790,7 +848,7 @@ void b_sm_begin(const Batch& b, hipStream_t s) {
 }
 
 void b_moved_stats(const Batch& b, int cur, hipStream_t s) {
-  kb_stats_accum<false, true, true, true><<<b.st_blocks, 256, 0, s>>>(b, cur);
+  kb_stats_accum<false, true, true><<<b.st_blocks, 256, 0, s>>>(b, cur);
 }
 
 void b_scan(const Batch& b, int which, hipStream_t s) {
