@@ -40,7 +40,8 @@ __global__ void kb_stats_clear(Batch b, int bbox) {
 // labels above max_spix) into sst.
 //
 // A workgroup covers a kStatTile x kStatTile tile: wave w its rows 8w..8w+7,
-// lane l the four pixels at column 4*(l&7) of row l>>3.  For each distinct
+// pixel slot j of lane l column l&31 of row 8w + 2j + (l>>5) (each slot's
+// loads cover two whole 128-byte row segments).  For each distinct
 // label of a wave (usually one to three) the lanes sum their pixels of that
 // label, DPP folds the sums into lane 63, and lane 63 issues one set of
 // atomics.  Position moments are summed relative to the tile origin in int32
@@ -62,8 +63,8 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
   const int tiles_x = (W + kStatTile - 1) / kStatTile;
   const int tx0 = (tile % tiles_x) * kStatTile, ty0 = (tile / tiles_x) * kStatTile;
   const int lane = threadIdx.x & 63;
-  const int lx = 4 * (lane & 7), ly = 8 * (threadIdx.x >> 6) + (lane >> 3);
-  const int y = ty0 + ly;
+  const int dx = lane & 31, wrow = 8 * (threadIdx.x >> 6) + (lane >> 5);
+  const int x = tx0 + dx;
   const int* labs = MOVED ? F.halves : F.seg[cur];
   LabelStats st = MOVED ? F.sst : F.st;
   const int nlab = MOVED ? F.nsm : F.nbuf;
@@ -73,27 +74,31 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
   float col[4][3];
   int inv[4];
   unsigned pend = 0;
+  // the labels of the lane's four pixels are loaded before any is used
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int x = tx0 + lx + j;
-    k[j] = -1;
+    const int y = ty0 + wrow + 2 * j;
+    k[j] = (x < W && y < H) ? labs[y * W + x] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int y = ty0 + wrow + 2 * j;
+    const int t = y * W + x;
+    const int v = k[j];
     inv[j] = 0;
     col[j][0] = col[j][1] = col[j][2] = 0.f;
-    if (x < W && y < H) {
-      const int t = y * W + x;
-      const int v = labs[t];
-      // the pass before a split also starts the halves map (a copy of the
-      // labels; the split search then writes the moving parts)
-      if (BBOX) F.halves[t] = v;
-      if (v >= lo && v < nlab) {
-        k[j] = v;
-        pend |= 1u << j;
-        const float* p = F.img + 3 * t;
-        col[j][0] = p[0];
-        col[j][1] = p[1];
-        col[j][2] = p[2];
-        if (shifted) inv[j] = shifted[t] < 0;
-      }
+    // the pass before a split also starts the halves map (a copy of the
+    // labels; the split search then writes the moving parts)
+    if (BBOX && x < W && y < H) F.halves[t] = v;
+    if (v >= lo && v < nlab) {
+      pend |= 1u << j;
+      const float* p = F.img + 3 * t;
+      col[j][0] = p[0];
+      col[j][1] = p[1];
+      col[j][2] = p[2];
+      if (shifted) inv[j] = shifted[t] < 0;
+    } else {
+      k[j] = -1;
     }
   }
   while (true) {
@@ -104,31 +109,31 @@ __global__ void __launch_bounds__(256) kb_stats_accum(Batch b, int cur) {
     const int u = __builtin_amdgcn_readlane(first, leader);
     int cnt = 0, sdx = 0, sdy = 0, sxx = 0, sxy = 0, syy = 0, ninv = 0;
     int64_t S[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
-    int mnx = 0x7fffffff, mxx = -1;
+    int mny = 0x7fffffff, mxy = -1;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       if (((pend >> j) & 1u) && k[j] == u) {
         pend &= ~(1u << j);
-        const int dx = lx + j;
+        const int dy = wrow + 2 * j;
         cnt += 1;
         ninv += inv[j];
         sdx += dx;
-        sdy += ly;
+        sdy += dy;
         sxx += dx * dx;
-        sxy += dx * ly;
-        syy += ly * ly;
+        sxy += dx * dy;
+        syy += dy * dy;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           S[c] += fx_color(col[j][c]);
           if (SQ) Q[c] += fx_square(col[j][c]);
         }
         if (BBOX) {
-          mnx = min(mnx, tx0 + dx);
-          mxx = max(mxx, tx0 + dx);
+          mny = min(mny, ty0 + dy);
+          mxy = max(mxy, ty0 + dy);
         }
       }
     }
-    int mny = cnt ? y : 0x7fffffff, mxy = cnt ? y : -1;
+    int mnx = cnt ? x : 0x7fffffff, mxx = cnt ? x : -1;
     if (shifted) ninv = wave_sum_to63(ninv);
     cnt = wave_sum_to63(cnt);
     sdx = wave_sum_to63(sdx);
