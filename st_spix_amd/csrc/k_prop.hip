@@ -387,9 +387,27 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
         // N, S, W, E
         const int nbv[4] = {mine && y > 0 ? seg[idx - W] : -1, mine && y < H - 1 ? seg[idx + W] : -1,
                             mine && x > 0 ? seg[idx - 1] : -1, mine && x < W - 1 ? seg[idx + 1] : -1};
+        // empty 4-neighbours not yet queued join the next frontier: their
+        // queue claims go out first (they depend only on the neighbours'
+        // labels), then the candidates' centers, all in flight together
+        const int nbi[4] = {idx - W, idx + W, idx - 1, idx + 1};
+        const bool ok[4] = {mine && y > 0, mine && y < H - 1, mine && x > 0, mine && x < W - 1};
+        uint32_t prev[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+          prev[d] = (ok[d] && nbv[d] == -1) ? atomicOr(&queued[nbi[d] >> 5], 1u << (nbi[d] & 31))
+                                            : 0xffffffffu;
+        double cx[4], cy[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int L = nbv[j] >= 0 ? nbv[j] : 0;
+          cx[j] = F.centers[2 * L];
+          cy[j] = F.centers[2 * L + 1];
+        }
         if (mine) {
           float best = -999999.0f;
           int lab = -1;
+#pragma unroll
           for (int j = 0; j < 4; j++) {
             const int L = nbv[j];
             if (L < 0) continue;
@@ -397,10 +415,8 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
             for (int u = 0; u < j; u++) dup |= nbv[u] == L;
             if (dup) continue;
             // isotropic_space, fill_missing.cu:140-153
-            float dx = (float)(((double)(1.0f * (float)x) - F.centers[2 * L]) /
-                               (double)(1.0f * (float)W));
-            float dy = (float)(((double)(1.0f * (float)y) - F.centers[2 * L + 1]) /
-                               (double)(1.0f * (float)H));
+            float dx = (float)(((double)(1.0f * (float)x) - cx[j]) / (double)(1.0f * (float)W));
+            float dy = (float)(((double)(1.0f * (float)y) - cy[j]) / (double)(1.0f * (float)H));
             float sim = -dx * dx - dy * dy;
             if (sim > best) {
               best = sim;
@@ -409,18 +425,9 @@ __global__ void __launch_bounds__(kFillThreads) kp_fill_all(Pro p) {
           }
           seg[idx] = lab;
         }
-        // empty 4-neighbours not yet queued join the next frontier, in the
-        // list of their stage: N and S neighbours flip the row parity, W and
-        // E the column parity
-        const int nbi[4] = {idx - W, idx + W, idx - 1, idx + 1};
-        const bool ok[4] = {mine && y > 0, mine && y < H - 1, mine && x > 0, mine && x < W - 1};
+        // the appends, in the list of each neighbour's stage: N and S
+        // neighbours flip the row parity, W and E the column parity
         const bool ycell = (y & 1) != 0;  // stage 1: a (0,1) cell; else (1,0)
-        // the four queue claims go out together, then the appends
-        uint32_t prev[4];
-#pragma unroll
-        for (int d = 0; d < 4; d++)
-          prev[d] = (ok[d] && nbv[d] == -1) ? atomicOr(&queued[nbi[d] >> 5], 1u << (nbi[d] & 31))
-                                            : 0xffffffffu;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
           const bool take = !((prev[d] >> (nbi[d] & 31)) & 1u);
