@@ -224,7 +224,7 @@ class _OutBatch:
         parts = torch.split(self.buf[:self.total], sizes)
         out = []
         j = 0
-        for (cap, H, W), n in zip(self.frames, ns):
+        for i, ((cap, H, W), n) in enumerate(zip(self.frames, ns)):
             p = ParamSet()
             for name, dt, w in _lib.PARAM_FIELDS:
                 v = parts[j].view(_DTYPES[dt])
@@ -232,8 +232,28 @@ class _OutBatch:
                 j += 2
             spix = parts[j].view(torch.int32).view(H, W)
             j += 2
+            # the struct of these arrays as the input of the stream's next
+            # frame (valid while the dict still holds these same tensors)
+            ps = self.params_struct(i)
+            ps.cap = int(n)
+            object.__setattr__(p, "_struct", (ps, tuple(p.values())))
             out.append((spix, p))
         return out
+
+
+def _cached_struct(p: Params, n: int) -> Optional[_lib.ParamsStruct]:
+    """The struct made with a batched call's outputs, if p still holds those
+    very tensors (no per-field checks needed), else None."""
+    c = getattr(p, "__dict__", {}).get("_struct")
+    if c is None:
+        return None
+    ps, tensors = c
+    if ps.cap != n or len(p) != len(tensors):
+        return None
+    for (name, _, _), t in zip(_lib.PARAM_FIELDS, tensors):
+        if p.get(name) is not t:
+            return None
+    return ps
 
 
 def params_struct(p: Params, cap: Optional[int] = None) -> _lib.ParamsStruct:
@@ -535,7 +555,7 @@ class BatchRunner:
             imgs[i] = img.data_ptr()
             flows[i] = flow.data_ptr()
             sprev[i] = spix_prev.data_ptr()
-            prevs[i] = params_struct(params_prev, np_)
+            prevs[i] = _cached_struct(params_prev, np_) or params_struct(params_prev, np_)
             nprev[i] = np_
         ob = _OutBatch(frames, items[0][1].device)
         for i in range(n):
