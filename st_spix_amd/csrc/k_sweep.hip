@@ -19,8 +19,10 @@ __device__ __forceinline__ void joint_b(const float* imgC, int x, int y, const S
   const int d0 = x >= r.cx ? x - r.cx : x - r.fx;  // == (int)((double)x - mean x)
   const int d1 = y >= r.cy ? y - r.cy : y - r.fy;
   res = res - (x0 * x0 + x1 * x1 + x2 * x2) / sigma2_app;
-  res = res - (float)(d0 * d0) * r.isig[0] - (float)(d1 * d1) * r.isig[2] -
-        (float)(2 * d0 * d1) * r.isig[1];
+  // products of the exactly converted offsets: each is the rounding of the
+  // same exact integer product the integer form converts
+  const float f0 = (float)d0, f1 = (float)d1;
+  res = res - (f0 * f0) * r.isig[0] - (f1 * f1) * r.isig[2] - ((f0 + f0) * f1) * r.isig[1];
   res = res - r.logdet;
   res = res - beta * neigh_neq;
   res = res - (float)(100 * (r.count <= 10));
@@ -129,7 +131,30 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   for (int i = tid; i < (TW * TH + 31) / 32; i += NT) touched[i] = 0u;
   const int tx = tile % tiles_x, ty = tile / tiles_x;
   const int x0 = tx * TW - R, y0 = ty * TH - R;
-  {
+  if constexpr ((NT % LW) * 8 <= NT) {
+    // all loads of the tile are issued before the first LDS store: each
+    // reads a clamped in-frame address, cells outside the frame get -1.
+    // A thread keeps one column and steps over rows.
+    constexpr int RP = NT / LW, NL = (LH + RP - 1) / RP;
+    const int c = tid % LW, r0 = tid / LW;
+    const int gx = x0 + c;
+    const int cx = min(max(gx, 0), W - 1);
+    const bool colin = (unsigned)gx < (unsigned)W;
+    int v[NL];
+    if (r0 < RP) {
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int gy = y0 + r0 + k * RP;
+        const int cy = min(max(gy, 0), H - 1);
+        if (r0 + k * RP < LH) v[k] = seg_in[cy * W + cx];
+      }
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int ly = r0 + k * RP, gy = y0 + ly;
+        if (ly < LH) lab[ly * LW + c] = (colin && (unsigned)gy < (unsigned)H) ? v[k] : -1;
+      }
+    }
+  } else {
     // all loads of the tile are issued before the first LDS store: each
     // reads a clamped in-frame address, cells outside the frame get -1
     constexpr int NL = (LH * LW + NT - 1) / NT;
@@ -166,33 +191,51 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
       // rows by0 .. by0 + bh - 1 and columns R - rb .. R + TW + rb - 1
       const int by0 = R - rb, bh = TH + 2 * rb;
       const int cx0 = (R - rb) >> 2, ncx = ((R + TW + rb + 3) >> 2) - cx0;
-      for (int i = tid; i < ncx * bh; i += NT) {
-        const int ly = by0 + i / ncx, x = 4 * (cx0 + i % ncx);
-        const int li = ly * LW + x;
-        const int4 c4 = *reinterpret_cast<const int4*>(lab + li);
-        const int4 u4 = *reinterpret_cast<const int4*>(lab + li - LW);
-        const int4 d4 = *reinterpret_cast<const int4*>(lab + li + LW);
-        const int lft = lab[li - 1], rgt = lab[li + 4];
-        const int C[4] = {c4.x, c4.y, c4.z, c4.w};
-        const int U[4] = {u4.x, u4.y, u4.z, u4.w};
-        const int D[4] = {d4.x, d4.y, d4.z, d4.w};
-        const int L[4] = {lft, c4.x, c4.y, c4.z};
-        const int Rt[4] = {c4.y, c4.z, c4.w, rgt};
-        const int gy = y0 + ly;
-        const bool rowin = gy >= 0 && gy < H;
-        const bool rowedge = gy < 1 || gy >= H - 1;
+      // A thread keeps one chunk column and steps over rows, so everything
+      // that depends on x alone is computed once per sweep: for cell j of the
+      // chunk, whether it lies in the frame, on the frame's edge, and how far
+      // outside the tile's columns (negative inside).  A cell of phase p is
+      // in the phase's region when its distance outside the tile (the larger
+      // of the column and row distances) is at most rb - p.
+      const int rpf = NT / ncx;
+      const int cc = tid % ncx, r0 = tid / ncx;
+      const int x = 4 * (cx0 + cc);
+      bool colin[4], coledge[4];
+      int dxo[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int lx = x + j, gx = x0 + lx;
-          const bool diff = (C[j] != U[j]) | (C[j] != D[j]) | (C[j] != L[j]) | (C[j] != Rt[j]);
-          const bool edge = rowedge || gx < 1 || gx >= W - 1;
-          const bool inside = rowin && gx >= 0 && gx < W;
-          const int p = ((j & 1) << 1) | (gy & 1);  // x0 + x is even
-          const int rp = 4 * (S - 1 - s) + (3 - p);
-          const bool inreg = lx >= R - rp && lx < R + TW + rp && ly >= R - rp && ly < R + TH + rp;
-          if (inside && (edge || diff) && inreg) {
-            const int pos = atomicAdd(&cnt[p], 1);
-            list[p][pos] = (unsigned short)(li + j);
+      for (int j = 0; j < 4; j++) {
+        const int lx = x + j, gx = x0 + lx;
+        colin[j] = (unsigned)gx < (unsigned)W;
+        coledge[j] = gx < 1 || gx >= W - 1;
+        dxo[j] = max(R - lx, lx - (R + TW - 1));
+      }
+      if (r0 < rpf) {
+        for (int ly = by0 + r0; ly < by0 + bh; ly += rpf) {
+          const int li = ly * LW + x;
+          const int4 c4 = *reinterpret_cast<const int4*>(lab + li);
+          const int4 u4 = *reinterpret_cast<const int4*>(lab + li - LW);
+          const int4 d4 = *reinterpret_cast<const int4*>(lab + li + LW);
+          const int lft = lab[li - 1], rgt = lab[li + 4];
+          const int C[4] = {c4.x, c4.y, c4.z, c4.w};
+          const int U[4] = {u4.x, u4.y, u4.z, u4.w};
+          const int D[4] = {d4.x, d4.y, d4.z, d4.w};
+          const int L[4] = {lft, c4.x, c4.y, c4.z};
+          const int Rt[4] = {c4.y, c4.z, c4.w, rgt};
+          const int gy = y0 + ly;
+          const bool rowin = (unsigned)gy < (unsigned)H;
+          const bool rowedge = gy < 1 || gy >= H - 1;
+          const int par = gy & 1;  // x0 + x is even: phase p = 2 (j & 1) + par
+          const int dyo = max(R - ly, ly - (R + TH - 1));
+          const int th[2] = {rb - par, rb - 2 - par};
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const bool diff = (C[j] != U[j]) | (C[j] != D[j]) | (C[j] != L[j]) | (C[j] != Rt[j]);
+            const bool inreg = max(dxo[j], dyo) <= th[j & 1];
+            if (rowin && colin[j] && (rowedge || coledge[j] || diff) && inreg) {
+              const int p = ((j & 1) << 1) | par;
+              const int pos = atomicAdd(&cnt[p], 1);
+              list[p][pos] = (unsigned short)(li + j);
+            }
           }
         }
       }
