@@ -194,23 +194,28 @@ def test_config3_16_streams_served_equals_plain():
 
 def fp64_means(img, spix, n):
     """FP64 recount of per-label colour means, positions and sample
-    covariances from the final labels."""
+    covariances from the final labels (numpy bincount on the host: sums in
+    float64, in pixel order; returned as tensors on the labels' device)."""
     H, W = spix.shape
-    lab = spix.flatten().long()
-    cnt = torch.zeros(n, dtype=torch.float64, device=img.device).index_add_(
-        0, lab, torch.ones_like(lab, dtype=torch.float64))
-    col = torch.zeros(n, 3, dtype=torch.float64, device=img.device).index_add_(
-        0, lab, img.reshape(-1, 3).double())
-    yy, xx = torch.meshgrid(torch.arange(H, device=img.device, dtype=torch.float64),
-                            torch.arange(W, device=img.device, dtype=torch.float64),
-                            indexing="ij")
-    pos = torch.stack([xx.flatten(), yy.flatten()], -1)
-    ps = torch.zeros(n, 2, dtype=torch.float64, device=img.device).index_add_(0, lab, pos)
-    mom = torch.stack([pos[:, 0] ** 2, pos[:, 0] * pos[:, 1], pos[:, 1] ** 2], -1)
-    ms = torch.zeros(n, 3, dtype=torch.float64, device=img.device).index_add_(0, lab, mom)
-    mu = ps / cnt[:, None]
-    samp = ms / cnt[:, None] - torch.stack([mu[:, 0] ** 2, mu[:, 0] * mu[:, 1], mu[:, 1] ** 2], -1)
-    return cnt, col / cnt[:, None], mu, samp
+    lab = spix.flatten().long().cpu().numpy()
+    px = img.reshape(-1, 3).double().cpu().numpy()
+    yy, xx = np.meshgrid(np.arange(H, dtype=np.float64), np.arange(W, dtype=np.float64),
+                         indexing="ij")
+    xx, yy = xx.ravel(), yy.ravel()
+
+    def sums(w):
+        return np.bincount(lab, weights=w, minlength=n)[:n]
+
+    cnt = np.bincount(lab, minlength=n)[:n].astype(np.float64)
+    col = np.stack([sums(px[:, c]) for c in range(3)], -1)
+    ps = np.stack([sums(xx), sums(yy)], -1)
+    ms = np.stack([sums(xx * xx), sums(xx * yy), sums(yy * yy)], -1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mu = ps / cnt[:, None]
+        samp = ms / cnt[:, None] - np.stack([mu[:, 0] ** 2, mu[:, 0] * mu[:, 1], mu[:, 1] ** 2], -1)
+        mean = col / cnt[:, None]
+    dev = spix.device
+    return tuple(torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (cnt, mean, mu, samp))
 
 
 def test_config4_fp32_against_fp64_report():
