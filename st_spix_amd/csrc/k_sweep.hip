@@ -116,7 +116,12 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   __shared__ int cnt[4];
   __shared__ uint32_t lutb[8];
   __shared__ uint32_t touched[(TW * TH + 31) / 32];  // tile cells whose label moved
-  const int f = frame_of(b.off.tile[SHAPE], b.nf, blockIdx.x);
+  // Workgroup i runs on XCD i % 8; each XCD takes a contiguous run of the
+  // tiles instead (raster order inside a frame), so neighbouring tiles share
+  // its L2 for their overlapping label rows, image lines and records.
+  const int nbk = gridDim.x, xcd = blockIdx.x & 7, q8 = nbk >> 3, r8 = nbk & 7;
+  const int bid = xcd * q8 + min(xcd, r8) + (blockIdx.x >> 3);
+  const int f = frame_of(b.off.tile[SHAPE], b.nf, bid);
   const FrameDesc& F = b.fr[f];
   const int H = F.H, W = F.W;
   const float* __restrict__ img = F.img;
@@ -125,7 +130,7 @@ __global__ void __launch_bounds__(NT) kb_sweep(Batch b, int cur, SimpleLut lut) 
   const SweepRec* __restrict__ rec = F.rec;
   const float sigma2_app = F.sigma2_app, potts = F.potts;
   const int tiles_x = (W + TW - 1) / TW;
-  const int tile = blockIdx.x - b.off.tile[SHAPE][f];
+  const int tile = bid - b.off.tile[SHAPE][f];
   const int tid = threadIdx.x;
   if (tid < 8) lutb[tid] = lut.bits[tid];
   for (int i = tid; i < (TW * TH + 31) / 32; i += NT) touched[i] = 0u;
